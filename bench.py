@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the tile processor hot path on one MI355X.
+
+One step = one 4-camera 2592 x 1936 Bayer frame through
+  convert_correct_tiles (forward CLT + aberration kernel + phase shift,
+  all 78 408 tiles x 4 cameras x 3 colors) and
+  correlate2D (6 camera pairs per tile, 15 x 15 output tiles),
+the drop-in staged path (the CLT is written to and read back from HBM).
+Inputs are synthetic (seeded), already resident in HBM when timing starts.
+
+Prints one JSON line: frames/s, the HBM roofline of the dominant kernel
+(algorithmic bytes / its HIP-event duration), the error against the FP64
+oracle on a sample of tiles, and the CPU oracle timed on a bounded sample.
+
+  python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FAT_ZERO = 0.05
+SCALES = (0.25, 0.25, 0.5)
+RADIUS = 7
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(g, num_corr):
+    """Compulsory unique bytes at the reference's buffer boundaries."""
+    cams, colors = 4, 3
+    images = cams * g.img_width * g.img_height * 4
+    kernels = cams * g.kernels_vert * g.kernels_hor * colors * 256 * 4
+    koffs = cams * g.kernels_vert * g.kernels_hor * colors * 32
+    tasks = g.num_tiles * 40
+    clt = cams * g.num_tiles * colors * 256 * 4
+    n = 2 * RADIUS + 1
+    convert = images + kernels + koffs + tasks + clt
+    correlate = clt + num_corr * 4 + num_corr * n * n * 4
+    return convert, correlate
+
+
+def err_vs_fp64(cfg, fb, idx, sample_tiles):
+    """Max relative error (max|x - x64| / max|x64|) of the GPU outputs on a
+    sample of tiles against the FP64 oracle."""
+    import torch
+    from oracle import oracle as orc
+    g = cfg["geom"]
+    tasks = cfg["tasks"][sample_tiles]
+    clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], tasks)
+    torch.cuda.synchronize()
+    worst_clt = 0.0
+    for cam in range(4):
+        got = fb.clt[cam].view(-1, 3, 256)[torch.as_tensor(sample_tiles, device=fb.device)]
+        got = got.cpu().numpy().astype(np.float64)
+        ref = clt64[cam].reshape(-1, 3, 256)[sample_tiles]
+        for color in range(3):
+            e = np.abs(got[:, color] - ref[:, color]).max() / np.abs(ref[:, color]).max()
+            worst_clt = max(worst_clt, float(e))
+    sel = np.nonzero(np.isin(idx >> 8, sample_tiles))[0]
+    ref = orc.correlate2D(g, clt64, 3, SCALES, FAT_ZERO, idx[sel], RADIUS)
+    got = fb.corrs[torch.as_tensor(sel, device=fb.device)].cpu().numpy().astype(np.float64)
+    e_corr = float(np.abs(got[:, :ref.shape[1]] - ref).max() / np.abs(ref).max())
+    n = 2 * RADIUS + 1
+    pk_got = got[:, :n * n].argmax(1)
+    pk_ref = ref.argmax(1)
+    return {"clt": worst_clt, "corr": e_corr,
+            "argmax_mismatch": int((pk_got != pk_ref).sum()), "sample_tiles": int(len(sample_tiles))}
+
+
+def cpu_baseline(cfg, idx, rows, reps=3):
+    """FP32 oracle (OpenMP over tiles) on the first `rows` tile rows, best of
+    `reps` runs."""
+    from oracle import oracle as orc
+    g = cfg["geom"]
+    cores = orc.num_threads()
+    n = rows * g.tilesx
+    tasks = cfg["tasks"][:n]
+    sel = idx[(idx >> 8) < n]
+    # warm-up on a few tiles (page in the inputs)
+    orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], tasks[:g.tilesx],
+                              dtype=np.float32)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        clt = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], tasks,
+                                        dtype=np.float32)
+        orc.correlate2D(g, clt, 3, SCALES, FAT_ZERO, sel, RADIUS)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    frac = n / g.num_tiles
+    return {"value": frac / dt, "unit": "frames/s", "cores": int(cores), "kind": "port",
+            "sample": f"{rows} of {g.tilesy} tile rows ({n} tiles, {len(sel)} pairs), "
+                      f"convert + correlate, FP32 oracle, best of {reps}: {dt:.3f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-rows", type=int, default=242, help="tile rows for the CPU baseline (0: skip)")
+    ap.add_argument("--err-tiles", type=int, default=512)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tile_processor_gpu_amd import synth, tp
+    from tile_processor_gpu_amd.geometry import corr_indices
+
+    t0 = time.time()
+    cfg = synth.config_full(seed=args.seed)
+    g = cfg["geom"]
+    idx = corr_indices(g, cfg["tasks"])
+    log(f"inputs {g.img_width}x{g.img_height}, {g.num_tiles} tiles, {len(idx)} pairs "
+        f"({time.time() - t0:.1f} s)")
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
+    fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+    torch.cuda.synchronize()
+
+    def step():
+        fb.convert()
+        fb.correlate(3, SCALES, FAT_ZERO)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} done")
+
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record()
+        fb.convert()
+        ev[k][1].record()
+        fb.correlate(3, SCALES, FAT_ZERO)
+        ev[k][2].record()
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    total_s = t_end - t_start
+    conv_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    corr_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    log(f"timed {K} steps: {1e3 * total_s / K:.3f} ms/step; convert {conv_ms:.3f} ms, "
+        f"correlate {corr_ms:.3f} ms")
+
+    conv_b, corr_b = algorithmic_bytes(g, len(idx))
+    kernels = {
+        "convert_correct_tiles": {"ms": conv_ms, "bytes": conv_b,
+                                  "GBps": conv_b / (conv_ms * 1e-3) / 1e9},
+        "correlate2D": {"ms": corr_ms, "bytes": corr_b, "GBps": corr_b / (corr_ms * 1e-3) / 1e9},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+                "traffic": None}
+    frame_bytes = conv_b + corr_b
+    stage = {k: {"ms": round(v["ms"], 4), "GBps": round(v["GBps"], 1)} for k, v in kernels.items()}
+
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(g.num_tiles, size=min(args.err_tiles, g.num_tiles), replace=False))
+    err = err_vs_fp64(cfg, fb, idx, sample)
+    log(f"err vs fp64: {err}")
+
+    cpu = None
+    if args.cpu_rows > 0:
+        cpu = cpu_baseline(cfg, idx, args.cpu_rows)
+        log(f"cpu baseline: {cpu}")
+
+    value = K / total_s
+    out = {
+        "metric": "quad-frame CLT+corr frames/s, % HBM roofline, err vs FP64 (plain vs MFMA/NN)",
+        "value": round(value, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total_s / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Bayer frames, Gaussian-PSF kernels, planar disparity 0-20 px)",
+        "config": {"workload": "plain FP32 staged CLT + aberration + 6-pair correlation (r=7)",
+                   "cameras": 4, "width": g.img_width, "height": g.img_height,
+                   "tiles": g.num_tiles, "pairs": int(len(idx)), "fat_zero": FAT_ZERO,
+                   "method": "plain"},
+        "roofline": roofline,
+        "frame_hbm_GBps": round(frame_bytes / total_s * K / 1e9, 1),
+        "stages": stage,
+        "err_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * tp.h - C ABI of the MI355X (gfx950) tile processor, libtp.so.
+ *
+ * Each entry point is a host function that validates its arguments and
+ * enqueues hand-written HIP kernels on the caller's stream.  Names, argument
+ * order and argument meaning follow the reference's extern "C" kernels
+ * (src/TileProcessor.cuh, mfkiwl/tile_processor_gpu); the compile-time
+ * geometry of the reference (IMG_WIDTH, KERNELS_HOR, ...) is passed at run
+ * time in a tp_geometry, and every call takes a stream.
+ *
+ * Pointer conventions (same as the reference):
+ *   - "float **" arguments are DEVICE arrays of TP_NUM_CAMS device pointers
+ *     (one buffer per camera), exactly what the reference kernels receive.
+ *   - every other pointer is a device pointer unless stated otherwise.
+ *   - strides are in floats, not bytes.
+ * No entry point allocates device memory or synchronises the stream, so all
+ * of them can be captured into a hipGraph.
+ *
+ * Return value: TP_OK (0) or a negative TP_E* code; tp_last_error() gives a
+ * text for the last failure on the calling thread.
+ */
+#ifndef TP_H
+#define TP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- fixed constants (src/TileProcessor.cuh:44-71, src/dtt8x8.cuh:48-52) ---- */
+#define TP_DTT_SIZE           8   /* 8x8 DCT/DST, 16x16 lapped windows      */
+#define TP_NUM_CAMS           4
+#define TP_NUM_COLORS         3   /* Bayer R, B, G (in that order)          */
+#define TP_NUM_PAIRS          6
+#define TP_KERNELS_LSTEP      4   /* aberration kernel cell = 16x16 px      */
+#define TP_CORR_NTILE_SHIFT   8   /* list entry = tile << 8 | low bits      */
+#define TP_TASK_CORR_BITS     4   /* tp_task.task bits 4..9: pairs          */
+#define TP_TASK_TEXTURE_BITS  0xf /* tp_task.task bits 0..3: N,E,S,W        */
+#define TP_LIST_TEXTURE_BIT   7   /* texture list entry flag                */
+#define TP_MAX_CORR_RADIUS    7   /* 15x15 correlation tiles                */
+
+/* Bayer color indices (src/TileProcessor.cuh:155-157); GR/BG mosaic with red
+ * at (row 0, col 1). */
+#define TP_BAYER_RED   0
+#define TP_BAYER_BLUE  1
+#define TP_BAYER_GREEN 2
+
+/* Camera pairs of the correlation list, pair index -> (cam1, cam2)
+ * (src/TileProcessor.cuh:484-490):
+ * 0:(0,1) 1:(2,3) 2:(0,2) 3:(1,3) 4:(0,3) 5:(2,1) */
+
+/* ---- status codes ---- */
+#define TP_OK        0
+#define TP_EINVAL   -1   /* bad argument, nothing was enqueued  */
+#define TP_ELAUNCH  -2   /* kernel launch failed                */
+#define TP_EHIP     -3   /* other HIP runtime error             */
+
+/* Tile task, binary identical to the reference (40 bytes,
+ * src/TileProcessor.cuh:179-186).
+ *   task : bits 0-3 texture neighbours N,E,S,W; bits 4-9 the 6 pairs;
+ *          0 = nothing to do.
+ *   txy  : tx | ty << 16.
+ *   xy   : tile centre in each camera, pixels (image x, y). */
+typedef struct tp_task {
+    int32_t task;
+    int32_t txy;
+    float   xy[TP_NUM_CAMS][2];
+} tp_task;
+
+/* Per kernel cell and color offsets, binary identical to the reference's
+ * CltExtra (src/TileProcessor.cuh:187-196). */
+typedef struct tp_clt_extra {
+    float data_x;    /* kernel is relative to this displacement X (px) */
+    float data_y;
+    float center_x;  /* informative only                               */
+    float center_y;
+    float dxc_dx;    /* d(data_x)/dx per pixel from the cell centre      */
+    float dxc_dy;
+    float dyc_dx;
+    float dyc_dy;
+} tp_clt_extra;
+
+/* Run-time replacement for IMG_WIDTH, IMG_HEIGHT, KERNELS_HOR and
+ * KERNELS_VERT (src/TileProcessor.cuh:47-50).  Derived:
+ *   tilesx = img_width / 8, tilesy = img_height / 8,
+ *   tilesya = (tilesy + 3) & ~3.
+ * img_width and img_height must be positive multiples of 16. */
+typedef struct tp_geometry {
+    int32_t img_width;
+    int32_t img_height;
+    int32_t kernels_hor;
+    int32_t kernels_vert;
+} tp_geometry;
+
+typedef void *tp_stream_t; /* a hipStream_t; NULL = the null stream */
+
+/* Text of the last error on this thread ("" if none). */
+const char *tp_last_error(void);
+
+/* Library version string. */
+const char *tp_version(void);
+
+/* Replaces the reference's cudaMemcpyToSymbol on its low-pass tables
+ * (src/TileProcessor.cuh:421-476).  which: 0..3 = lpf_data[R,B,G,mono],
+ * 4 = lpf_rb_corr, 5 = lpf_corr.  host_lpf: 64 floats in HOST memory,
+ * [vertical freq][horizontal freq], or NULL to restore the built-in table.
+ * Applies to the current device; not stream-ordered (call it outside any
+ * graph capture).  The built-in tables are the closed-form Gaussian
+ * responses the reference's tables hold. */
+int tp_set_lpf(int which, const float *host_lpf);
+
+/* Forward CLT of every camera and color of each task tile, with per-tile
+ * aberration correction (CLT-domain kernel multiply) and fractional-pixel
+ * phase rotation.  Replaces convert_correct_tiles
+ * (src/TileProcessor.cuh:1859-1937) and convertCorrectTile (:3106-3634).
+ *   gpu_kernel_offsets: [cam] -> tp_clt_extra[kernels_vert][kernels_hor][3]
+ *   gpu_kernels       : [cam] -> float[kernels_vert][kernels_hor][3][4][8][8]
+ *   gpu_images        : [cam] -> float[img_height][dstride] Bayer image
+ *   gpu_tasks         : tp_task[num_tiles]; task == 0 tiles are skipped
+ *   gpu_clt           : [cam] -> float[tilesy][tilesx][3][4][8][8] (output)
+ *   lpf_mask          : bit c applies lpf_data[c] to color c (reference: 0)
+ * Tasks whose (tx, ty) lies outside the tile grid are skipped.
+ * Border tiles: windows are placed as the reference does; pixels outside
+ * the image are read from the nearest pixel of the same Bayer color. */
+int tp_convert_correct_tiles(const tp_geometry *geom,
+                             float **gpu_kernel_offsets,
+                             float **gpu_kernels,
+                             float **gpu_images,
+                             const tp_task *gpu_tasks,
+                             float **gpu_clt,
+                             size_t dstride,
+                             int num_tiles,
+                             int lpf_mask,
+                             tp_stream_t stream);
+
+/* Pairwise phase correlation in the CLT domain, back to a pixel-domain
+ * (2r+1)^2 tile per list entry.  Replaces correlate2D
+ * (src/TileProcessor.cuh:1176-1443).
+ *   gpu_corr_indices : int[num_corr_tiles], tile << 8 | pair (pair 0..5)
+ *   gpu_corrs        : row i (corr_stride floats apart) receives entry i
+ *   corr_radius      : 0..7
+ * Entries with pair > 5 or a tile outside the grid are skipped (row left
+ * untouched). */
+int tp_correlate2D(const tp_geometry *geom,
+                   float **gpu_clt,
+                   int colors,
+                   float scale0,
+                   float scale1,
+                   float scale2,
+                   float fat_zero,
+                   size_t num_corr_tiles,
+                   const int *gpu_corr_indices,
+                   size_t corr_stride,
+                   int corr_radius,
+                   float *gpu_corrs,
+                   tp_stream_t stream);
+
+/* Inverse CLT of one color of one camera, overlap-added into a plane, one of
+ * the 4 tile-parity passes.  Replaces imclt_rbg
+ * (src/TileProcessor.cuh:2588-2712).
+ *   gpu_clt : float[tilesy][tilesx][3][4][8][8] of ONE camera
+ *   gpu_rbg : 3 color planes of (img_height + 8) rows x dstride floats;
+ *             tile (tx,ty) lands at row 8*ty, column 8*tx of its plane.
+ * Pass (v_offset, h_offset) = (0,0) initialises the tiles it covers, the
+ * other three accumulate.  The reference's debug writes (tile (0,0) set to
+ * 100, tile (161,111) doubled) are not reproduced. */
+int tp_imclt_rbg(const tp_geometry *geom,
+                 const float *gpu_clt,
+                 float *gpu_rbg,
+                 int color,
+                 int v_offset,
+                 int h_offset,
+                 size_t dstride,
+                 tp_stream_t stream);
+
+/* Whole-frame variant: for each camera, all colors, all 4 passes in order.
+ * gpu_rbg: [cam] -> planes as in tp_imclt_rbg (device pointer array). */
+int tp_imclt_rbg_all(const tp_geometry *geom,
+                     float **gpu_clt,
+                     float **gpu_rbg,
+                     size_t dstride,
+                     tp_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TP_H */

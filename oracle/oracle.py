@@ -1,0 +1,154 @@
+"""ctypes binding of the CPU oracle (liboracle.so built from tp_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the
+benchmark's CPU-baseline leg as the checker and the timed CPU reference,
+never by the product.  All arrays are host numpy arrays.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_VP = ctypes.c_void_p
+
+
+class CGeometry(ctypes.Structure):
+    _fields_ = [("img_width", ctypes.c_int32), ("img_height", ctypes.c_int32),
+                ("kernels_hor", ctypes.c_int32), ("kernels_vert", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.oracle_num_threads.restype = ctypes.c_int
+        _lib.oracle_table.restype = ctypes.c_int
+    return _lib
+
+
+def _cgeo(geom):
+    return CGeometry(geom.img_width, geom.img_height, geom.kernels_hor, geom.kernels_vert)
+
+
+def _ptrs(arrs):
+    return (_VP * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def num_threads(n=0):
+    return lib().oracle_num_threads(int(n))
+
+
+def table(kind):
+    """0: dtt matrices [4][8][8]; 1: fold rind/rcs/rss [3][16]; 2: windows
+    hw, hw2, hwsq, hwsqi [4][8]; 3: lpf [6][64]; 4: alpha fade [16][256]."""
+    sizes = {0: (4, 8, 8), 1: (3, 16), 2: (4, 8), 3: (6, 64), 4: (16, 256)}
+    out = np.zeros(int(np.prod(sizes[kind])), np.float64)
+    lib().oracle_table(int(kind), out.ctypes.data_as(_VP))
+    return out.reshape(sizes[kind])
+
+
+def set_lpf(which, tab=None):
+    a = None if tab is None else np.ascontiguousarray(tab, np.float32).reshape(64)
+    lib().oracle_set_lpf(int(which), None if a is None else a.ctypes.data_as(_VP))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def convert_tile(geom, koffs_cam, kernels_cam, image, dstride, color, cx, cy, lpf_mask=0,
+                 dtype=np.float64):
+    """One tile, one camera, one color -> [4][8][8]."""
+    q = np.zeros((4, 8, 8), dtype)
+    ko, ke, im = _f32(koffs_cam), _f32(kernels_cam), _f32(image)
+    fn = lib().oracle_convert_tile_f64 if dtype == np.float64 else lib().oracle_convert_tile_f32
+    g = _cgeo(geom)
+    fn(ctypes.byref(g), ko.ctypes.data_as(_VP), ke.ctypes.data_as(_VP), im.ctypes.data_as(_VP),
+       ctypes.c_size_t(dstride), ctypes.c_int(color), ctypes.c_int(lpf_mask), ctypes.c_float(cx),
+       ctypes.c_float(cy), q.ctypes.data_as(_VP))
+    return q
+
+
+def convert_correct_tiles(geom, koffs, kernels, images, tasks, dstride=None, lpf_mask=0,
+                          dtype=np.float64, clt=None):
+    """Oracle of convert_correct_tiles.  koffs/kernels/images: per-camera
+    arrays; tasks: TASK_DTYPE array.  Returns 4 arrays [tilesy*tilesx*3*256]."""
+    dstride = dstride or geom.img_width
+    ko = [_f32(a) for a in koffs]
+    ke = [_f32(a) for a in kernels]
+    im = [_f32(a) for a in images]
+    if clt is None:
+        clt = [np.zeros(geom.clt_floats, dtype) for _ in range(4)]
+    t = np.ascontiguousarray(tasks)
+    fn = (lib().oracle_convert_correct_tiles_f64 if dtype == np.float64
+          else lib().oracle_convert_correct_tiles_f32)
+    g = _cgeo(geom)
+    fn(ctypes.byref(g), _ptrs(ko), _ptrs(ke), _ptrs(im), t.ctypes.data_as(_VP), _ptrs(clt),
+       ctypes.c_size_t(dstride), ctypes.c_int(len(t)), ctypes.c_int(lpf_mask))
+    return clt
+
+
+def correlate2D(geom, clt, colors, scales, fat_zero, idx, radius, stride=None, dtype=None):
+    """Oracle of correlate2D; clt: 4 arrays (float32 or float64, selects the
+    variant).  Returns [len(idx)][stride]."""
+    dtype = dtype or clt[0].dtype
+    clt = [np.ascontiguousarray(c, dtype) for c in clt]
+    n = 2 * radius + 1
+    stride = stride or n * n
+    idx = np.ascontiguousarray(idx, np.int32)
+    out = np.zeros((len(idx), stride), dtype)
+    fn = lib().oracle_correlate2D_f64 if dtype == np.float64 else lib().oracle_correlate2D_f32
+    g = _cgeo(geom)
+    fn(ctypes.byref(g), _ptrs(clt), ctypes.c_int(colors), ctypes.c_float(scales[0]),
+       ctypes.c_float(scales[1]), ctypes.c_float(scales[2]), ctypes.c_float(fat_zero),
+       ctypes.c_size_t(len(idx)), idx.ctypes.data_as(_VP), ctypes.c_size_t(stride),
+       ctypes.c_int(radius), out.ctypes.data_as(_VP))
+    return out
+
+
+def imclt_tile(q, horizontal_first=1):
+    """Inverse CLT of one [4][8][8] tile -> [16][16] (no accumulation)."""
+    q = np.ascontiguousarray(q, np.float64)
+    out = np.zeros((16, 16))
+    lib().oracle_imclt_tile_f64(q.ctypes.data_as(_VP), out.ctypes.data_as(_VP),
+                                ctypes.c_int(horizontal_first))
+    return out
+
+
+def imclt_rbg(geom, clt_cam, rbg, color, v_offset, h_offset, dstride):
+    """Oracle of one imclt_rbg pass, in place on rbg (3 planes of
+    (H + 8) x dstride, dtype of rbg selects the variant)."""
+    clt_cam = np.ascontiguousarray(clt_cam, rbg.dtype)
+    fn = lib().oracle_imclt_rbg_f64 if rbg.dtype == np.float64 else lib().oracle_imclt_rbg_f32
+    g = _cgeo(geom)
+    fn(ctypes.byref(g), clt_cam.ctypes.data_as(_VP), rbg.ctypes.data_as(_VP), ctypes.c_int(color),
+       ctypes.c_int(v_offset), ctypes.c_int(h_offset), ctypes.c_size_t(dstride))
+    return rbg
+
+
+def imclt_frame(geom, clt, dstride=None, dtype=np.float64):
+    """All cameras, colors and the 4 passes in reference order, planes
+    zero-initialised -> list of 4 arrays [3][H + 8][dstride]."""
+    dstride = dstride or geom.img_width + 8
+    out = []
+    for cam in range(4):
+        rbg = np.zeros((3, geom.img_height + 8, dstride), dtype)
+        for color in range(3):
+            for v in range(2):
+                for h in range(2):
+                    imclt_rbg(geom, clt[cam], rbg, color, v, h, dstride)
+        out.append(rbg)
+    return out

@@ -1,0 +1,57 @@
+"""ctypes loader for libtp.so (the C ABI of include/tp.h).
+
+The library is built in-tree by `python -m tile_processor_gpu_amd.build`.
+There is no fallback: a missing library is an error.
+"""
+import ctypes
+import os
+
+from .geometry import CGeometry
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtp.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_SZ = ctypes.c_size_t
+_G = ctypes.POINTER(CGeometry)
+
+# name -> argument types (return type is int unless listed in _RESTYPES)
+PROTOTYPES = {
+    "tp_last_error": [],
+    "tp_version": [],
+    "tp_set_lpf": [_I, _P],
+    "tp_convert_correct_tiles": [_G, _P, _P, _P, _P, _P, _SZ, _I, _I, _P],
+    "tp_correlate2D": [_G, _P, _I, _F, _F, _F, _F, _SZ, _P, _SZ, _I, _P, _P],
+    "tp_imclt_rbg": [_G, _P, _P, _I, _I, _I, _SZ, _P],
+    "tp_imclt_rbg_all": [_G, _P, _P, _SZ, _P],
+}
+_RESTYPES = {"tp_last_error": ctypes.c_char_p, "tp_version": ctypes.c_char_p}
+
+_lib = None
+
+
+class TPError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TPError(f"{LIB_PATH} is not built; run python -m tile_processor_gpu_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argt in PROTOTYPES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().tp_last_error().decode()
+        raise TPError(f"{name} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,334 @@
+// k_convert.hpp - forward CLT with aberration correction (plain FP32).
+//
+// Replaces convert_correct_tiles / convertCorrectTile
+// (src/TileProcessor.cuh:1859-1937, :3106-3634).
+//
+// Work decomposition (gfx950, wave64): a "unit" is one (tile, camera, color)
+// 16x16 -> 4 x 8x8 transform.  8 lanes own a unit (lane = one row or column
+// of the 8x8 folded block), so a wavefront runs 8 units side by side.  A
+// workgroup is 3 wavefronts, one per Bayer color, over 2 tiles x 4 cameras;
+// it loops over consecutive tile pairs of a contiguous chunk.
+//
+// Per unit:
+//   1. lane = folded column k gathers its 8 (R, B) or 16 (G) Bayer samples,
+//      applies the shifted sine windows with the fold signs;
+//   2. vertical DCT-IV in registers;
+//   3. 8x8 transpose through LDS (lane = folded row l);
+//   4. horizontal DCT-IV (and DST-IV for G) in registers;
+//   5. the missing quadrants are replicated from the Bayer symmetry
+//      (register reversal + one DPP mirror across the unit's lanes);
+//   6. bicomplex multiply by the aberration kernel (2 x float4 loads per
+//      quadrant), horizontal then vertical phase rotation, optional LPF;
+//   7. 2 x float4 stores per quadrant.
+#pragma once
+
+#include "tp_common.hpp"
+
+namespace tp {
+
+// per-lane window coefficients: w_s(n) = P(n) cos(pi s/16) + Q(n) sin(pi s/16)
+// for n = 0..15 (w_s(n) = sin(pi (n + 1/2 + s)/16)).
+__device__ __forceinline__ void win_pq(int n, float &P, float &Q) {
+    const int i = n < 8 ? n : 15 - n;
+    const int j = n < 8 ? 7 - n : n - 8;
+    float hi = 0.f, hj = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {  // runtime index into the 8-entry table
+        hi = (i == t) ? kTab.hw[t] : hi;
+        hj = (j == t) ? kTab.hw[t] : hj;
+    }
+    P = hi;
+    Q = n < 8 ? hj : -hj;
+}
+
+struct ConvertArgs {
+    Geo g;
+    const float *const *koffs;
+    const float *const *kernels;
+    const float *const *images;
+    const tp_task *tasks;
+    float *const *clt;
+    size_t dstride;
+    int num_tiles;
+    int lpf_mask;
+    int pairs_per_block;
+};
+
+struct LpfTab {
+    float v[6][64];
+};
+constexpr LpfTab make_lpf_tab() {
+    LpfTab t{};
+    for (int w = 0; w < 6; w++)
+        for (int i = 0; i < 64; i++) t.v[w][i] = kTab.lpf[w][i];
+    return t;
+}
+// low-pass tables: lpf_data[R,B,G,mono], lpf_rb_corr, lpf_corr
+__constant__ LpfTab c_lpf = make_lpf_tab();
+
+template <int COLOR>
+__device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c, float *lds_s,
+                                             int li, int cam, int task_num, bool valid) {
+    constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
+    const Geo &g = a.g;
+    const tp_task *tk = a.tasks + task_num;
+    const int txy = tk->txy;
+    const float cx = tk->xy[cam][0], cy = tk->xy[cam][1];
+
+    // (1) kernel cell, :3129-3134
+    int ktx = (int)__builtin_rintf(fmaf(cx, 0.0625f, 1.0f));
+    int kty = (int)__builtin_rintf(fmaf(cy, 0.0625f, 1.0f));
+    ktx = min(g.KH - 1, max(0, ktx));
+    kty = min(g.KV - 1, max(0, kty));
+    const float kdx = (cx - (float)(ktx << 4)) + 8.0f;
+    const float kdy = (cy - (float)(kty << 4)) + 8.0f;
+    const size_t kfi = (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + COLOR;
+    const float *ex = a.koffs[cam] + kfi * 8;
+    const float offx = fmaf(ex[5], kdy, fmaf(ex[4], kdx, ex[0]));
+    const float offy = fmaf(ex[7], kdy, fmaf(ex[6], kdx, ex[1]));
+    const float *kern = a.kernels[cam] + kfi * 256;
+
+    // (2) integer top-left, residual shift, :3182-3247
+    const float px = (cx - 8.0f) - offx;
+    const float py = (cy - 8.0f) - offy;
+    int itlx = (int)floorf(px + 0.5f);
+    int itly = (int)floorf(py + 0.5f);
+    if (itlx < 0) itlx &= 1;
+    if (itlx >= g.W - 8) itlx = (itlx & 1) + (g.W - 10);
+    if (itly < 0) itly &= 1;
+    if (itly >= g.H - 8) itly = (itly & 1) + (g.H - 10);
+    const float sh = ((float)(itlx + 8) - cx) + offx;
+    const float sv = ((float)(itly + 8) - cy) + offy;
+    float ahs, ahc, avs, avc;
+    sincospif(sh * 0.0625f, &ahs, &ahc);
+    sincospif(sv * 0.0625f, &avs, &avc);
+
+    // (3) windows.  Vertical: all 16 source rows (compile-time n).
+    float wv[16];
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+        const float P = n < 8 ? kTab.hw[n] : kTab.hw[15 - n];
+        const float Q = n < 8 ? kTab.hw[7 - n] : -kTab.hw[n - 8];
+        wv[n] = (float)fold_rcs(n) * fmaf(P, avc, Q * avs);
+    }
+    // Horizontal: this lane's two source columns (folded column li).
+    const int ca = li >= 4 ? li - 4 : 11 - li;   // fold_src_a
+    const int cb = li >= 4 ? 11 - li : 12 + li;  // fold_src_b
+    float Pa, Qa, Pb, Qb;
+    win_pq(ca, Pa, Qa);
+    win_pq(cb, Pb, Qb);
+    const float wa = fmaf(Pa, ahc, Qa * ahs), wb = fmaf(Pb, ahc, Qb * ahs);
+    // column of each local parity, with signed cosine / sine windows
+    const int pa = ca & 1;
+    int colp[2];
+    float whc[2], whs[2];
+    colp[pa] = ca;
+    colp[pa ^ 1] = cb;
+    whc[pa] = (ca < 4 ? 1.f : -1.f) * wa;
+    whc[pa ^ 1] = (cb < 4 ? 1.f : -1.f) * wb;
+    whs[pa] = (ca < 12 ? 1.f : -1.f) * wa;
+    whs[pa ^ 1] = (cb < 12 ? 1.f : -1.f) * wb;
+
+    const float *img = a.images[cam];
+    const size_t ds = a.dstride;
+    auto pix = [&](int r, int c) -> float {
+        int y = itly + r, x = itlx + c;
+        y = min(y, g.H - 2 + (y & 1));
+        x = min(x, g.W - 2 + (x & 1));
+        return img[(size_t)y * ds + x];
+    };
+
+    // (4) fold: f[l] = folded (l, li)
+    float f[8], fs[8];
+    if (!color1) {
+        const int pc = (itlx & 1) ^ (1 ^ color0);
+        const int pr = (itly & 1) ^ color0;
+        const int c = colp[pc];
+        const float wc = whc[pc];
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            const bool use_a = (ra & 1) == pr;
+            const int r = use_a ? ra : rb;
+            const float w = use_a ? wv[ra] : wv[rb];
+            f[l] = pix(r, c) * w * wc;
+        }
+    } else {
+        // G sites: (row + col + itlx + itly) even; both source rows used
+        const int S = (itlx + itly) & 1;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            const int pca = (ra + S) & 1, pcb = (rb + S) & 1;
+            const float da = pix(ra, colp[pca]) * wv[ra];
+            const float db = pix(rb, colp[pcb]) * wv[rb];
+            f[l] = fmaf(da, whc[pca], db * whc[pcb]);
+            fs[l] = fmaf(da, whs[pca], db * whs[pcb]);
+        }
+    }
+
+    // (5) vertical DCT-IV (column li)
+    dtt8<0>(f);
+    if (color1) dtt8<0>(fs);
+
+    // (6) transpose through LDS: lane li writes column li as row li of the
+    // staging block, then reads its row li (element k from lane k).
+    *(float4 *)(lds_c + li * 8) = make_float4(f[0], f[1], f[2], f[3]);
+    *(float4 *)(lds_c + li * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
+    if (color1) {
+        *(float4 *)(lds_s + li * 8) = make_float4(fs[0], fs[1], fs[2], fs[3]);
+        *(float4 *)(lds_s + li * 8 + 4) = make_float4(fs[4], fs[5], fs[6], fs[7]);
+    }
+    wave_lds_sync();
+    float cc[8], sc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cc[k] = lds_c[k * 8 + li];
+    if (color1) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) sc[k] = lds_s[k * 8 + li];
+    }
+    wave_lds_sync();
+
+    // (7) horizontal DCT-IV (and DST-IV for G)
+    dtt8<0>(cc);
+    if (color1) dtt8<1>(sc);
+
+    // (8) replicate quadrants, :3457-3497.  Lane li is row l = li.
+    const float nd = (float)(1 - ((((itlx & 1) ^ (itly & 1) ^ (1 ^ color1))) << 1));
+    float q0[8], q1[8], q2[8], q3[8];
+    if (!color1) {
+        const float nh = (float)(1 - (((itlx & 1) ^ (1 ^ COLOR)) << 1));
+        float m[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) m[k] = mirror8(cc[k]);  // row 7-l
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            q0[k] = cc[k];
+            q1[k] = nh * cc[7 - k];
+            q3[k] = nd * m[7 - k];
+            q2[k] = (nd * nh) * m[k];
+        }
+    } else {
+        float m1[8], m2[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            m1[k] = mirror8(cc[k]);
+            m2[k] = mirror8(sc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            q0[k] = cc[k];
+            q1[k] = sc[k];
+            q3[k] = nd * m1[7 - k];
+            q2[k] = nd * m2[7 - k];
+        }
+    }
+
+    // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827
+    {
+        const float4 *kp = (const float4 *)(kern + li * 8);
+        float kq[4][8];
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            const float4 u = kp[qq * 16], v = kp[qq * 16 + 1];
+            kq[qq][0] = u.x; kq[qq][1] = u.y; kq[qq][2] = u.z; kq[qq][3] = u.w;
+            kq[qq][4] = v.x; kq[qq][5] = v.y; kq[qq][6] = v.z; kq[qq][7] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float c0 = q0[k], c1 = q1[k], c2 = q2[k], c3 = q3[k];
+            const float k0 = kq[0][k], k1 = kq[1][k], k2 = kq[2][k], k3 = kq[3][k];
+            q0[k] = fmaf(c3, k3, fmaf(-c2, k2, fmaf(-c1, k1, c0 * k0)));
+            q1[k] = fmaf(-c2, k3, fmaf(-c3, k2, fmaf(c0, k1, c1 * k0)));
+            q2[k] = fmaf(-c1, k3, fmaf(c0, k2, fmaf(-c3, k1, c2 * k0)));
+            q3[k] = fmaf(c0, k3, fmaf(c1, k2, fmaf(c2, k1, c3 * k0)));
+        }
+    }
+
+    // (10) phase rotations: horizontal by pi sh (2k+1)/16 (shiftTileHor
+    // :2715-2745), then vertical by pi sv (2l+1)/16 (shiftTileVert :2747-2776).
+    {
+        float er[8], ei[8];  // e^{i (2k+1) a}, a = pi sh / 16
+        const float e1r = ahc, e1i = ahs;
+        float e2r, e2i, e4r, e4i, e8r, e8i;
+        cmul(e1r, e1i, e1r, e1i, e2r, e2i);
+        cmul(e2r, e2i, e2r, e2i, e4r, e4i);
+        cmul(e4r, e4i, e4r, e4i, e8r, e8i);
+        er[0] = e1r; ei[0] = e1i;
+        cmul(e1r, e1i, e2r, e2i, er[1], ei[1]);        // 3
+        cmul(er[1], ei[1], e2r, e2i, er[2], ei[2]);    // 5
+        cmul(er[2], ei[2], e2r, e2i, er[3], ei[3]);    // 7
+#pragma unroll
+        for (int t = 0; t < 4; t++) cmul(er[t], ei[t], e8r, e8i, er[t + 4], ei[t + 4]);  // 9..15
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float ch = er[k], s = ei[k];
+            const float a0 = q0[k], b0 = q1[k], a2 = q2[k], b2 = q3[k];
+            q0[k] = fmaf(a0, ch, -b0 * s);
+            q1[k] = fmaf(a0, s, b0 * ch);
+            q2[k] = fmaf(a2, ch, -b2 * s);
+            q3[k] = fmaf(a2, s, b2 * ch);
+        }
+        float vs, vc;
+        sincospif(sv * (float)(2 * li + 1) * 0.0625f, &vs, &vc);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float a0 = q0[k], b0 = q2[k], a1 = q1[k], b1 = q3[k];
+            q0[k] = fmaf(a0, vc, -b0 * vs);
+            q2[k] = fmaf(a0, vs, b0 * vc);
+            q1[k] = fmaf(a1, vc, -b1 * vs);
+            q3[k] = fmaf(a1, vs, b1 * vc);
+        }
+    }
+    // (11) optional LPF, :3572-3590
+    if ((a.lpf_mask >> COLOR) & 1) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float lv = c_lpf.v[COLOR][li * 8 + k];
+            q0[k] *= lv; q1[k] *= lv; q2[k] *= lv; q3[k] *= lv;
+        }
+    }
+    // (12) store [q][l][k], :3600-3630
+    const int tx = txy & 0xfff, ty = txy >> 16;
+    if (valid && tx < g.TX && ty >= 0 && ty < g.TY) {  // out-of-grid tiles are skipped
+        float *dst = a.clt[cam] + ((size_t)(ty * g.TX + tx) * TP_NUM_COLORS + COLOR) * 256 + li * 8;
+        float4 *d4 = (float4 *)dst;
+        d4[0] = make_float4(q0[0], q0[1], q0[2], q0[3]);
+        d4[1] = make_float4(q0[4], q0[5], q0[6], q0[7]);
+        d4[16] = make_float4(q1[0], q1[1], q1[2], q1[3]);
+        d4[17] = make_float4(q1[4], q1[5], q1[6], q1[7]);
+        d4[32] = make_float4(q2[0], q2[1], q2[2], q2[3]);
+        d4[33] = make_float4(q2[4], q2[5], q2[6], q2[7]);
+        d4[48] = make_float4(q3[0], q3[1], q3[2], q3[3]);
+        d4[49] = make_float4(q3[4], q3[5], q3[6], q3[7]);
+    }
+}
+
+__global__ __launch_bounds__(192) void k_convert_correct_tiles(ConvertArgs a) {
+    __shared__ __align__(16) float lds[3][2][8][72];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int unit = lane >> 3;
+    const int li = lane & 7;
+    const int cam = unit & 3;
+    const int npairs = (a.num_tiles + 1) >> 1;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int p0 = blk * a.pairs_per_block;
+    const int p1 = min(npairs, p0 + a.pairs_per_block);
+    float *lds_c = &lds[wave][0][unit][0];
+    float *lds_s = &lds[wave][1][unit][0];
+    for (int p = p0; p < p1; p++) {
+        int task_num = p * 2 + (unit >> 2);
+        bool valid = task_num < a.num_tiles;
+        if (!valid) task_num = a.num_tiles - 1;
+        valid = valid && (a.tasks[task_num].task != 0);
+        if (wave == 0)
+            convert_unit<0>(a, lds_c, lds_s, li, cam, task_num, valid);
+        else if (wave == 1)
+            convert_unit<1>(a, lds_c, lds_s, li, cam, task_num, valid);
+        else
+            convert_unit<2>(a, lds_c, lds_s, li, cam, task_num, valid);
+    }
+}
+
+}  // namespace tp

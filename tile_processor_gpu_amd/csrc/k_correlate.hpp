@@ -1,0 +1,177 @@
+// k_correlate.hpp - pairwise phase correlation (plain FP32).
+//
+// Replaces correlate2D (src/TileProcessor.cuh:1176-1443) with
+// correlateAccumulateTiles (:2829-2890), normalizeTileAmplitude (:2915-2943)
+// and corrUnfoldTile (:2950-3008).
+//
+// A unit is one list entry (tile, pair); 8 lanes own a unit, 8 units per
+// wavefront, 4 wavefronts per workgroup, each workgroup a contiguous chunk of
+// the list (so the 6 pairs of a tile, adjacent in the usual tile-major list,
+// share one L2).  Per unit:
+//   1. lane = row l loads its row of the 4 quadrants of both cameras per
+//      color (2 x float4 per quadrant) and accumulates scale * conj(A) B;
+//      after color 1 (B) the sum is low-passed (lpf_rb_corr);
+//   2. amplitude normalisation with fat zero, lpf_corr;
+//   3. horizontal DCT-II / DST-II in registers, transpose through LDS,
+//      vertical DCT-II / DST-II (lane = column j);
+//   4. unfold to (2r+1)^2: column j combines its own column with column j-1
+//      (one DPP row shift) and writes output columns r+j and r-j.
+#pragma once
+
+#include "tp_common.hpp"
+
+namespace tp {
+
+struct CorrArgs {
+    float *const *clt;
+    int colors;
+    float scale[3];
+    float fat_zero;
+    int num_corr;
+    const int *idx;
+    size_t stride;
+    int radius;
+    float *corrs;
+    int ntiles;             // tiles per camera buffer (tilesx * tilesy)
+    int entries_per_block;  // multiple of 32
+};
+
+// value of lane - 1 within each row of 16 lanes (lane 0 of a row: 0)
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111 /* row_shr:1 */,
+                                                      0xf, 0xf, false));
+}
+
+__global__ __launch_bounds__(256) void k_correlate2D(CorrArgs a) {
+    __shared__ __align__(16) float lds[4][8][4][72];  // [wave][unit][quadrant][8x8+pad]
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int unit = lane >> 3;
+    const int li = lane & 7;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int e0 = blk * a.entries_per_block;
+    const int e1 = min(a.num_corr, e0 + a.entries_per_block);
+    const int r = a.radius, n = 2 * r + 1;
+    const float fz2 = a.fat_zero * a.fat_zero;
+
+    for (int eb = e0 + wave * 8; eb < e1; eb += 32) {
+        int e = eb + unit;
+        bool valid = e < e1;
+        if (!valid) e = e1 - 1;
+        const int v = a.idx[e];
+        int tile = v >> TP_CORR_NTILE_SHIFT;
+        int pair = v & ((1 << TP_CORR_NTILE_SHIFT) - 1);
+        if (pair >= TP_NUM_PAIRS || tile < 0 || tile >= a.ntiles) {  // skipped entries
+            valid = false;
+            pair = 0;
+            tile = 0;
+        }
+        const int cam1 = pair == 0 ? 0 : pair == 1 ? 2 : pair == 2 ? 0 : pair == 3 ? 1 : pair == 4 ? 0 : 2;
+        const int cam2 = pair == 0 ? 1 : pair == 1 ? 3 : pair == 2 ? 2 : pair == 3 ? 3 : pair == 4 ? 3 : 1;
+
+        float c0[8], c1[8], c2[8], c3[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) c0[k] = c1[k] = c2[k] = c3[k] = 0.f;
+
+        for (int color = 0; color < a.colors; color++) {
+            const size_t off = ((size_t)tile * TP_NUM_COLORS + color) * 256 + li * 8;
+            const float4 *pa = (const float4 *)(a.clt[cam1] + off);
+            const float4 *pb = (const float4 *)(a.clt[cam2] + off);
+            float A[4][8], B[4][8];
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) {
+                float4 u = pa[qq * 16], w = pa[qq * 16 + 1];
+                A[qq][0] = u.x; A[qq][1] = u.y; A[qq][2] = u.z; A[qq][3] = u.w;
+                A[qq][4] = w.x; A[qq][5] = w.y; A[qq][6] = w.z; A[qq][7] = w.w;
+                u = pb[qq * 16]; w = pb[qq * 16 + 1];
+                B[qq][0] = u.x; B[qq][1] = u.y; B[qq][2] = u.z; B[qq][3] = u.w;
+                B[qq][4] = w.x; B[qq][5] = w.y; B[qq][6] = w.z; B[qq][7] = w.w;
+            }
+            const float sc = a.scale[color];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float a0 = A[0][k], a1 = A[1][k], a2 = A[2][k], a3 = A[3][k];
+                const float b0 = B[0][k], b1 = B[1][k], b2 = B[2][k], b3 = B[3][k];
+                // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
+                const float r0 = fmaf(a3, b3, fmaf(a2, b2, fmaf(a1, b1, a0 * b0)));
+                const float r1 = fmaf(a2, b3, fmaf(-a3, b2, fmaf(a0, b1, -a1 * b0)));
+                const float r2 = fmaf(a1, b3, fmaf(a0, b2, fmaf(-a3, b1, -a2 * b0)));
+                const float r3 = fmaf(a0, b3, fmaf(-a1, b2, fmaf(-a2, b1, a3 * b0)));
+                c0[k] = fmaf(sc, r0, c0[k]);
+                c1[k] = fmaf(sc, r1, c1[k]);
+                c2[k] = fmaf(sc, r2, c2[k]);
+                c3[k] = fmaf(sc, r3, c3[k]);
+            }
+            if (color == 1) {  // R + B low-pass, :1263-1304
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const float lv = c_lpf.v[4][li * 8 + k];
+                    c0[k] *= lv; c1[k] *= lv; c2[k] *= lv; c3[k] *= lv;
+                }
+            }
+        }
+        // normalise with fat zero, then lpf_corr, :2915-2943, :1337-1347
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float s2 = fmaf(c3[k], c3[k], fmaf(c2[k], c2[k], fmaf(c1[k], c1[k], fmaf(c0[k], c0[k], fz2))));
+            const float s = __frsqrt_rn(s2) * c_lpf.v[5][li * 8 + k];
+            c0[k] *= s; c1[k] *= s; c2[k] *= s; c3[k] *= s;
+        }
+        // horizontal DCT-II (q0, q2) / DST-II (q1, q3) on row li
+        dtt8<2>(c0);
+        dtt8<3>(c1);
+        dtt8<2>(c2);
+        dtt8<3>(c3);
+        // transpose: row li -> LDS, read column li
+        float *L = &lds[wave][unit][0][0];
+        {
+            float *q[4] = {c0, c1, c2, c3};
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) {
+                *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
+                *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            c0[i] = L[0 * 72 + i * 8 + li];
+            c1[i] = L[1 * 72 + i * 8 + li];
+            c2[i] = L[2 * 72 + i * 8 + li];
+            c3[i] = L[3 * 72 + i * 8 + li];
+        }
+        wave_lds_sync();
+        // vertical DCT-II (q0, q1) / DST-II (q2, q3) on column li
+        dtt8<2>(c0);
+        dtt8<2>(c1);
+        dtt8<3>(c2);
+        dtt8<3>(c3);
+        // unfold, :2950-3008.  Lane li is column j; q1, q3 come from column j-1.
+        float p1[8], p3[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            p1[i] = lane_prev(c1[i]);
+            p3[i] = lane_prev(c3[i]);
+        }
+        const int j = li;
+        if (valid && j <= r) {
+            float *out = a.corrs + (size_t)e * a.stride;
+            const float s4 = 0.25f;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (i > r) break;
+                const float a0 = c0[i];
+                const float a1 = j ? p1[i] : 0.f;
+                const float a2 = i ? c2[i - 1] : 0.f;
+                const float a3 = (i && j) ? p3[i - 1] : 0.f;
+                const float t1 = a0 + a2, t2 = a0 - a2, u1 = a1 + a3, u2 = a1 - a3;
+                out[(r + i) * n + r + j] = s4 * (t1 + u1);
+                out[(r + i) * n + r - j] = s4 * (t1 - u1);
+                out[(r - i) * n + r + j] = s4 * (t2 + u2);
+                out[(r - i) * n + r - j] = s4 * (t2 - u2);
+            }
+        }
+    }
+}
+
+}  // namespace tp

@@ -1,0 +1,164 @@
+// k_imclt.hpp - inverse CLT of color planes with overlap-add (plain FP32).
+//
+// Replaces imclt_rbg (src/TileProcessor.cuh:2588-2712) and imclt
+// (:3700-3838).  A unit is one tile of one color plane; 8 lanes per unit:
+//   1. lane = row l loads row l of the 4 quadrants;
+//   2. horizontal DCT-IV (quadrants 0, 2) / DST-IV (1, 3) in registers;
+//   3. transpose through LDS (lane = column k);
+//   4. vertical DCT-IV (0, 1) / DST-IV (2, 3);
+//   5. unfold: lane k produces the two output columns that fold onto k,
+//      for all 16 rows, windowed by HWINDOW2(row) * HWINDOW2(col).
+//
+// k_imclt_pass reproduces one reference pass (RMW of the plane in HBM).
+// k_imclt_frame is output-stationary: a workgroup owns a 64x64 block of one
+// plane, runs the 4 parity passes of the (at most 9x9) tiles that touch it
+// with the sums kept in LDS, and writes the block once.
+#pragma once
+
+#include "tp_common.hpp"
+
+namespace tp {
+
+// HWINDOW2[i] = sin(pi (2i+1)/32) / 2 for a runtime i
+__device__ __forceinline__ float hw2_sel(int i) {
+    float h = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) h = (i == t) ? 0.5f * kTab.hw[t] : h;
+    return h;
+}
+
+// Steps 2-4 on one unit; q* in: row li, out: column li of each quadrant.
+__device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], float (&q2)[8],
+                                                float (&q3)[8], float *L, int li) {
+    dtt8<0>(q0);
+    dtt8<1>(q1);
+    dtt8<0>(q2);
+    dtt8<1>(q3);
+    float *q[4] = {q0, q1, q2, q3};
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) {
+        *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
+        *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        q0[i] = L[0 * 72 + i * 8 + li];
+        q1[i] = L[1 * 72 + i * 8 + li];
+        q2[i] = L[2 * 72 + i * 8 + li];
+        q3[i] = L[3 * 72 + i * 8 + li];
+    }
+    wave_lds_sync();
+    dtt8<0>(q0);
+    dtt8<0>(q1);
+    dtt8<1>(q2);
+    dtt8<1>(q3);
+}
+
+// Step 5: calls emit(row, col, weight, sum) for the 32 outputs of lane li.
+template <typename F>
+__device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (&t1)[8],
+                                             const float (&t2)[8], const float (&t3)[8], int li,
+                                             F &&emit) {
+    const int ca = li >= 4 ? li - 4 : 11 - li;
+    const int cb = li >= 4 ? 11 - li : 12 + li;
+    const float wca = hw2_sel(ca < 8 ? ca : 15 - ca);
+    const float wcb = hw2_sel(cb < 8 ? cb : 15 - cb);
+    const float cca = ca < 4 ? 1.f : -1.f, sca = ca < 12 ? 1.f : -1.f;
+    const float ccb = cb < 4 ? 1.f : -1.f, scb = cb < 12 ? 1.f : -1.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int rho = fold_rind(r);
+        const float cr = (float)fold_rcs(r), sr = (float)fold_rss(r);
+        const float A = fmaf(cr, t0[rho], sr * t2[rho]);
+        const float B = fmaf(cr, t1[rho], sr * t3[rho]);
+        const float wr = 0.5f * kTab.hw[r < 8 ? r : 15 - r];
+        emit(r, ca, wr * wca, fmaf(cca, A, sca * B));
+        emit(r, cb, wr * wcb, fmaf(ccb, A, scb * B));
+    }
+}
+
+__device__ __forceinline__ void load_rows(const float *src, int li, float (&q0)[8], float (&q1)[8],
+                                          float (&q2)[8], float (&q3)[8]) {
+    const float4 *p = (const float4 *)(src + li * 8);
+    float *q[4] = {q0, q1, q2, q3};
+#pragma unroll
+    for (int qq = 0; qq < 4; qq++) {
+        const float4 u = p[qq * 16], v = p[qq * 16 + 1];
+        q[qq][0] = u.x; q[qq][1] = u.y; q[qq][2] = u.z; q[qq][3] = u.w;
+        q[qq][4] = v.x; q[qq][5] = v.y; q[qq][6] = v.z; q[qq][7] = v.w;
+    }
+}
+
+// One reference pass: tiles of parity (v_off, h_off), 8 per wavefront.
+__global__ __launch_bounds__(64) void k_imclt_pass(Geo g, const float *clt, float *rbg, int color,
+                                                   int v_off, int h_off, size_t dstride) {
+    __shared__ __align__(16) float lds[8][4][72];
+    const int lane = threadIdx.x;
+    const int unit = lane >> 3, li = lane & 7;
+    const int th = (g.TX + (h_off ^ 1)) >> 1;
+    const int tv = (g.TY + (v_off ^ 1)) >> 1;
+    int n = blockIdx.x * 8 + unit;
+    const bool valid = n < th * tv;
+    if (!valid) n = 0;
+    const int ty = (n / th) * 2 + v_off, tx = (n % th) * 2 + h_off;
+    const bool first = (v_off | h_off) == 0;
+    float q0[8], q1[8], q2[8], q3[8];
+    load_rows(clt + ((size_t)(ty * g.TX + tx) * TP_NUM_COLORS + color) * 256, li, q0, q1, q2, q3);
+    imclt_transform(q0, q1, q2, q3, &lds[unit][0][0], li);
+    float *plane = rbg + dstride * (size_t)(g.H + 8) * color + (size_t)(ty * 8) * dstride + tx * 8;
+    if (valid) {
+        imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {
+            float *p = plane + (size_t)r * dstride + c;
+            *p = first ? w * d : fmaf(w, d, *p);
+        });
+    }
+}
+
+// Whole frame, output-stationary.  grid = (64x64 blocks, color, camera).
+__global__ __launch_bounds__(256) void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
+                                                     size_t dstride) {
+    __shared__ __align__(16) float acc[64][65];
+    __shared__ __align__(16) float lds[4][8][4][72];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int unit = lane >> 3, li = lane & 7;
+    const int PW = g.W + 8, PH = g.H + 8;
+    const int nbx = (PW + 63) / 64;
+    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
+    const int color = blockIdx.y, cam = blockIdx.z;
+    const int X0 = bx * 64, Y0 = by * 64;
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) acc[i >> 6][i & 63] = 0.f;
+    __syncthreads();
+    const float *cl = clt[cam];
+    for (int pass = 0; pass < 4; pass++) {
+        const int v = pass >> 1, h = pass & 1;
+        // tiles of this parity touching the block: ty in [8by-1, 8by+7]
+        const int ty0 = 8 * by - 1 + ((8 * by - 1 - v) & 1 ? 1 : 0);
+        const int tx0 = 8 * bx - 1 + ((8 * bx - 1 - h) & 1 ? 1 : 0);
+        const int nty = (8 * by + 7 - ty0) / 2 + 1, ntx = (8 * bx + 7 - tx0) / 2 + 1;
+        const int slot = wave * 8 + unit;
+        for (int s = slot; s < nty * ntx; s += 32) {
+            const int ty = ty0 + 2 * (s / ntx), tx = tx0 + 2 * (s % ntx);
+            const bool valid = ty >= 0 && ty < g.TY && tx >= 0 && tx < g.TX;
+            float q0[8], q1[8], q2[8], q3[8];
+            const int tyc = valid ? ty : 0, txc = valid ? tx : 0;
+            load_rows(cl + ((size_t)(tyc * g.TX + txc) * TP_NUM_COLORS + color) * 256, li, q0, q1, q2, q3);
+            imclt_transform(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
+            if (valid) {
+                const int oy = ty * 8 - Y0, ox = tx * 8 - X0;
+                imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {
+                    const int y = oy + r, x = ox + c;
+                    if ((unsigned)y < 64u && (unsigned)x < 64u) acc[y][x] = fmaf(w, d, acc[y][x]);
+                });
+            }
+        }
+        __syncthreads();
+    }
+    float *plane = rbg[cam] + dstride * (size_t)PH * color;
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+        const int y = Y0 + (i >> 6), x = X0 + (i & 63);
+        if (y < PH && x < PW) plane[(size_t)y * dstride + x] = acc[i >> 6][i & 63];
+    }
+}
+
+}  // namespace tp

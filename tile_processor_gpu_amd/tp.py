@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference kernels' calling convention over libtp.so.
+
+Each function here has the name, argument order and argument meaning of the
+reference kernel it replaces (src/TileProcessor.cuh), plus the run-time
+geometry and an optional stream.  Arguments the reference passes as
+`float **` (device arrays of per-camera device pointers) are torch int64
+device tensors built by `ptr_array`.  Work is enqueued on `stream` (default:
+torch's current stream) and nothing synchronises.
+
+All device memory here is torch's; the kernels are the HIP code of libtp.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import Geometry, NUM_CAMS, NUM_COLORS, TASK_DTYPE
+
+_VP = ctypes.c_void_p
+
+
+def _p(t):
+    return _VP(t.data_ptr()) if t is not None else _VP(0)
+
+
+def _stream(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    if isinstance(stream, torch.cuda.Stream):
+        return _VP(stream.cuda_stream)
+    return _VP(int(stream))
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise ValueError("tensor arguments must be on the GPU")
+
+
+def ptr_array(tensors):
+    """Device array of device pointers (the reference's `float **`)."""
+    _check_dev(*tensors)
+    return torch.tensor([t.data_ptr() for t in tensors], dtype=torch.int64,
+                        device=tensors[0].device)
+
+
+def version():
+    return _lib.lib().tp_version().decode()
+
+
+def set_lpf(which, table):
+    """Replace a low-pass table (reference: cudaMemcpyToSymbol on lpf_data,
+    lpf_rb_corr, lpf_corr).  which: 0..3 lpf_data[R,B,G,mono], 4 lpf_rb_corr,
+    5 lpf_corr; table: 64 floats [vertical][horizontal], or None for the
+    built-in table."""
+    if table is None:
+        _lib.call("tp_set_lpf", int(which), _VP(0))
+        return
+    a = np.ascontiguousarray(table, dtype=np.float32).reshape(64)
+    _lib.call("tp_set_lpf", int(which), a.ctypes.data_as(_VP))
+
+
+def convert_correct_tiles(geom, gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt,
+                          dstride, num_tiles, lpf_mask=0, stream=None):
+    """convert_correct_tiles (src/TileProcessor.cuh:1859-1937)."""
+    _check_dev(gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt)
+    g = geom.c()
+    _lib.call("tp_convert_correct_tiles", ctypes.byref(g), _p(gpu_kernel_offsets), _p(gpu_kernels),
+              _p(gpu_images), _p(gpu_tasks), _p(gpu_clt), int(dstride), int(num_tiles),
+              int(lpf_mask), _stream(stream))
+
+
+def correlate2D(geom, gpu_clt, colors, scale0, scale1, scale2, fat_zero, num_corr_tiles,
+                gpu_corr_indices, corr_stride, corr_radius, gpu_corrs, stream=None):
+    """correlate2D (src/TileProcessor.cuh:1176-1443)."""
+    _check_dev(gpu_clt, gpu_corr_indices, gpu_corrs)
+    g = geom.c()
+    _lib.call("tp_correlate2D", ctypes.byref(g), _p(gpu_clt), int(colors), float(scale0),
+              float(scale1), float(scale2), float(fat_zero), int(num_corr_tiles),
+              _p(gpu_corr_indices), int(corr_stride), int(corr_radius), _p(gpu_corrs),
+              _stream(stream))
+
+
+def imclt_rbg(geom, gpu_clt, gpu_rbg, color, v_offset, h_offset, dstride, stream=None):
+    """imclt_rbg (src/TileProcessor.cuh:2588-2712): one camera, one color,
+    one of the four tile-parity passes."""
+    _check_dev(gpu_clt, gpu_rbg)
+    g = geom.c()
+    _lib.call("tp_imclt_rbg", ctypes.byref(g), _p(gpu_clt), _p(gpu_rbg), int(color),
+              int(v_offset), int(h_offset), int(dstride), _stream(stream))
+
+
+def imclt_rbg_all(geom, gpu_clt, gpu_rbg, dstride, stream=None):
+    """All cameras, colors and passes in one call (gpu_clt, gpu_rbg: pointer arrays)."""
+    _check_dev(gpu_clt, gpu_rbg)
+    g = geom.c()
+    _lib.call("tp_imclt_rbg_all", ctypes.byref(g), _p(gpu_clt), _p(gpu_rbg), int(dstride),
+              _stream(stream))
+
+
+class FrameBuffers:
+    """Device buffers of one 4-camera frame in the reference layout, with the
+    pointer arrays the `float **` arguments need."""
+
+    def __init__(self, geom: Geometry, device="cuda:0", dstride=None, corr_radius=7,
+                 max_corr=None, corr_stride=None):
+        self.geom = geom
+        self.device = torch.device(device)
+        W, H = geom.img_width, geom.img_height
+        self.dstride = dstride or W
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self.images = [torch.zeros((H, self.dstride), **f32) for _ in range(NUM_CAMS)]
+        self.kernels = [torch.zeros(geom.kernel_floats, **f32) for _ in range(NUM_CAMS)]
+        self.koffs = [torch.zeros(geom.kernels_vert * geom.kernels_hor * NUM_COLORS * 8, **f32)
+                      for _ in range(NUM_CAMS)]
+        self.clt = [torch.zeros(geom.clt_floats, **f32) for _ in range(NUM_CAMS)]
+        self.p_images = ptr_array(self.images)
+        self.p_kernels = ptr_array(self.kernels)
+        self.p_koffs = ptr_array(self.koffs)
+        self.p_clt = ptr_array(self.clt)
+        self.tasks = None
+        self.num_tiles = 0
+        self.corr_radius = corr_radius
+        n = 2 * corr_radius + 1
+        self.corr_stride = corr_stride or n * n
+        self.max_corr = max_corr if max_corr is not None else geom.num_tiles * 6
+        self.corr_idx = torch.zeros(max(1, self.max_corr), dtype=torch.int32, device=self.device)
+        self.corrs = torch.zeros((max(1, self.max_corr), self.corr_stride), **f32)
+        self.num_corr = 0
+
+    def upload(self, images=None, kernels=None, koffs=None, tasks=None, corr_idx=None):
+        if images is not None:
+            for c in range(NUM_CAMS):
+                im = torch.from_numpy(np.ascontiguousarray(images[c], dtype=np.float32))
+                self.images[c][:, :im.shape[1]].copy_(im)
+        if kernels is not None:
+            for c in range(NUM_CAMS):
+                self.kernels[c].copy_(torch.from_numpy(
+                    np.ascontiguousarray(kernels[c], dtype=np.float32).reshape(-1)))
+        if koffs is not None:
+            for c in range(NUM_CAMS):
+                self.koffs[c].copy_(torch.from_numpy(
+                    np.ascontiguousarray(koffs[c], dtype=np.float32).reshape(-1)))
+        if tasks is not None:
+            t = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+            self.tasks = torch.from_numpy(t.view(np.uint8).copy()).to(self.device)
+            self.num_tiles = len(t)
+        if corr_idx is not None:
+            ci = np.ascontiguousarray(corr_idx, dtype=np.int32)
+            if len(ci) > self.max_corr:
+                raise ValueError("correlation list longer than max_corr")
+            self.corr_idx[:len(ci)].copy_(torch.from_numpy(ci))
+            self.num_corr = len(ci)
+
+    def convert(self, lpf_mask=0, stream=None):
+        convert_correct_tiles(self.geom, self.p_koffs, self.p_kernels, self.p_images, self.tasks,
+                              self.p_clt, self.dstride, self.num_tiles, lpf_mask, stream)
+
+    def correlate(self, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, stream=None):
+        correlate2D(self.geom, self.p_clt, colors, scales[0], scales[1], scales[2], fat_zero,
+                    self.num_corr, self.corr_idx, self.corr_stride, self.corr_radius, self.corrs,
+                    stream)
