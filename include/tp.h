@@ -40,6 +40,7 @@ extern "C" {
 #define TP_TASK_TEXTURE_BITS  0xf /* tp_task.task bits 0..3: N,E,S,W        */
 #define TP_LIST_TEXTURE_BIT   7   /* texture list entry flag                */
 #define TP_MAX_CORR_RADIUS    7   /* 15x15 correlation tiles                */
+#define TP_FAT_ZERO_WEIGHT    0.0001 /* added to texture port weights      */
 
 /* Bayer color indices (src/TileProcessor.cuh:155-157); GR/BG mosaic with red
  * at (row 0, col 1). */
@@ -181,6 +182,109 @@ int tp_imclt_rbg_all(const tp_geometry *geom,
                      float **gpu_clt,
                      float **gpu_rbg,
                      size_t dstride,
+                     tp_stream_t stream);
+
+/* Textures of a list of tiles.  Replaces textures_accumulate
+ * (src/TileProcessor.cuh:2207-2585): per tile the 4 cameras' CLT are
+ * low-passed (lpf_data), inverse transformed, debayered with shot-noise
+ * companding, combined by the 4-camera consensus, and written as
+ *   - non-overlapping tiles (if gpu_texture_tiles && texture_stride):
+ *     entry i -> gpu_texture_tiles + i * texture_stride, slices of 256
+ *     floats: colors, alpha (+ 4 port weights, colors + 1 rms if
+ *     keep_weights); and/or
+ *   - the alpha-faded overlap-add into a woi-relative RGBA image (if
+ *     gpu_texture_rbg && texture_rbg_stride; entries with texture mask
+ *     bits 0..3 == 0 are not added): colors + 1 slices of
+ *     (woi[3] + 1) * 8 rows of texture_rbg_stride floats.
+ *   gpu_texture_indices : tile << 8 | 1 << 7 | N,E,S,W mask
+ *   gpu_port_offsets    : float[4][2] camera port offsets
+ *   woi                 : device int[4] x, y, width, height (tiles)
+ * Tiles of one call must not overlap in the RGBA image (the reference runs
+ * one call per tile parity); is_lwir is unused, as in the reference. */
+int tp_textures_accumulate(const tp_geometry *geom,
+                           const int *woi,
+                           float **gpu_clt,
+                           size_t num_texture_tiles,
+                           const int *gpu_texture_indices,
+                           const float *gpu_port_offsets,
+                           int colors,
+                           int is_lwir,
+                           float min_shot,
+                           float scale_shot,
+                           float diff_sigma,
+                           float diff_threshold,
+                           float min_agree,
+                           float weight0,
+                           float weight1,
+                           float weight2,
+                           int dust_remove,
+                           int keep_weights,
+                           size_t texture_rbg_stride,
+                           float *gpu_texture_rbg,
+                           size_t texture_stride,
+                           float *gpu_texture_tiles,
+                           tp_stream_t stream);
+
+/* Texture list of a task list.  Replaces prepare_texture_list
+ * (src/TileProcessor.cuh:1643-1713) and its child kernels (:1716-1854),
+ * which the reference launches with CUDA dynamic parallelism; here they are
+ * stream-ordered kernels with device-side counts.
+ *   gpu_tasks          : bits 0..3 of each textured task are REPLACED by the
+ *                        N,E,S,W neighbours that are textured too (a tile
+ *                        with none drops out of the list, as in the reference)
+ *   gpu_texture_indices: int[tilesx * tilesya]; 4 parity sub-lists of
+ *                        tilesx * tilesya / 4 entries, inner tiles from the
+ *                        start, border tiles from the end of each; the
+ *                        first tilesx * height entries also hold the tile
+ *                        occupancy map while the list is built
+ *   num_texture_tiles  : device int[8]: [2 * parity + border]
+ *   woi                : device int[4] x, y, width, height of the textured
+ *                        tiles
+ *   width, height      : extent of the occupancy map to clear (<= tilesx,
+ *                        <= tilesy)
+ * Entry order within a sub-list is unspecified (as in the reference). */
+int tp_prepare_texture_list(const tp_geometry *geom,
+                            tp_task *gpu_tasks,
+                            int num_tiles,
+                            int *gpu_texture_indices,
+                            int *num_texture_tiles,
+                            int *woi,
+                            int width,
+                            int height,
+                            tp_stream_t stream);
+
+/* Texture list + RGBA image in one stream-ordered sequence.  Replaces
+ * generate_RBGA (src/TileProcessor.cuh:1446-1613): prepare_texture_list,
+ * then clearing the RGBA image of the woi, then the 8 textures_accumulate
+ * passes (4 tile parities x inner/border) into gpu_texture_tiles, an RGBA
+ * image of colors + 1 slices with (woi height + 1) * 8 rows of
+ * texture_rbga_stride floats (allocate (tilesy + 1) * 8 rows per slice).
+ * No host synchronisation: pass sizes are read on the device.  keep_weights
+ * is ignored (the reference's passes use 0). */
+int tp_generate_RBGA(const tp_geometry *geom,
+                     tp_task *gpu_tasks,
+                     int num_tiles,
+                     int *gpu_texture_indices,
+                     int *num_texture_tiles,
+                     int *woi,
+                     int width,
+                     int height,
+                     float **gpu_clt,
+                     const float *gpu_port_offsets,
+                     int colors,
+                     int is_lwir,
+                     float min_shot,
+                     float scale_shot,
+                     float diff_sigma,
+                     float diff_threshold,
+                     float min_agree,
+                     float weight0,
+                     float weight1,
+                     float weight2,
+                     int dust_remove,
+                     int keep_weights,
+                     size_t texture_rbga_stride,
+                     float *gpu_texture_tiles,
                      tp_stream_t stream);
 
 #ifdef __cplusplus

@@ -152,3 +152,135 @@ def imclt_frame(geom, clt, dstride=None, dtype=np.float64):
                     imclt_rbg(geom, clt[cam], rbg, color, v, h, dstride)
         out.append(rbg)
     return out
+
+
+class TexParams:
+    """Texture parameters of textures_accumulate (reference harness values
+    by default, src/dtt8x8.cu:791-814)."""
+
+    def __init__(self, colors=3, min_shot=10.0, scale_shot=3.0, diff_sigma=1.5,
+                 diff_threshold=10.0, min_agree=3.0, weights=(0.294118, 0.117647, 0.588235),
+                 dust_remove=1, keep_weights=0):
+        self.colors, self.min_shot, self.scale_shot = colors, min_shot, scale_shot
+        self.diff_sigma, self.diff_threshold, self.min_agree = diff_sigma, diff_threshold, min_agree
+        self.weights, self.dust_remove, self.keep_weights = weights, dust_remove, keep_weights
+
+    def cargs(self):
+        f = ctypes.c_float
+        return [ctypes.c_int(self.colors), f(self.min_shot), f(self.scale_shot),
+                f(self.diff_sigma), f(self.diff_threshold), f(self.min_agree),
+                f(self.weights[0]), f(self.weights[1]), f(self.weights[2]),
+                ctypes.c_int(self.dust_remove), ctypes.c_int(self.keep_weights)]
+
+    def slices(self, tiles=True):
+        full = self.colors + 1 + 4 + self.colors + 1
+        return full if (self.keep_weights or not tiles) else self.colors + 1
+
+
+PORTS = np.array([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]], np.float32)
+
+
+def textures_accumulate(geom, woi, clt, idx, params, rbga=None, rbga_stride=0, tiles=None,
+                        tex_stride=0, ports=PORTS):
+    """Oracle of textures_accumulate; rbga / tiles are updated in place (their
+    dtype must match clt's)."""
+    dtype = clt[0].dtype
+    clt = [np.ascontiguousarray(c, dtype) for c in clt]
+    idx = np.ascontiguousarray(idx, np.int32)
+    w = np.ascontiguousarray(woi if woi is not None else [0, 0, 0, 0], np.int32)
+    po = np.ascontiguousarray(ports, np.float32)
+    fn = lib().oracle_textures_accumulate_f64 if dtype == np.float64 else lib().oracle_textures_accumulate_f32
+    g = _cgeo(geom)
+    fn(ctypes.byref(g), w.ctypes.data_as(_VP), _ptrs(clt), ctypes.c_size_t(len(idx)),
+       idx.ctypes.data_as(_VP), po.ctypes.data_as(_VP), *params.cargs(),
+       ctypes.c_size_t(rbga_stride), _VP(rbga.ctypes.data if rbga is not None else 0),
+       ctypes.c_size_t(tex_stride), _VP(tiles.ctypes.data if tiles is not None else 0))
+
+
+def prepare_texture_list(geom, tasks, width=None, height=None):
+    """Returns (mutated tasks, indices [tilesx * tilesya], num[8], woi[4])."""
+    t = np.array(tasks, copy=True)
+    ind = np.zeros(geom.tilesx * geom.tilesya, np.int32)
+    num = np.zeros(8, np.int32)
+    woi = np.zeros(4, np.int32)
+    g = _cgeo(geom)
+    lib().oracle_prepare_texture_list(ctypes.byref(g), t.ctypes.data_as(_VP), ctypes.c_int(len(t)),
+                                      ind.ctypes.data_as(_VP), num.ctypes.data_as(_VP),
+                                      woi.ctypes.data_as(_VP),
+                                      ctypes.c_int(width or geom.tilesx),
+                                      ctypes.c_int(height or geom.tilesy))
+    return t, ind, num, woi
+
+
+def sub_lists(geom, ind, num):
+    """The 8 (parity x inner/border) texture lists as in generate_RBGA's passes."""
+    q = geom.tilesx * (geom.tilesya >> 2)
+    out = []
+    for p in range(8):
+        border = p >> 2
+        ntt = int(num[((p & 3) << 1) + border])
+        off = (p & 3) * q + (q - ntt if border else 0)
+        out.append(ind[off:off + ntt].copy())
+    return out
+
+
+def generate_rbga(geom, tasks, clt, params, rbga_stride=None, ports=PORTS):
+    """Oracle of generate_RBGA: texture list, then the 8 passes into a zeroed
+    RGBA image of (woi_h + 1) * 8 rows per slice.  Returns (tasks, woi, num,
+    lists, rbga [colors + 1][rows][rbga_stride])."""
+    t, ind, num, woi = prepare_texture_list(geom, tasks)
+    rbga_stride = rbga_stride or (geom.tilesx + 1) * 8
+    rows = (woi[3] + 1) * 8
+    rbga = np.zeros((params.colors + 1, rows, rbga_stride), clt[0].dtype)
+    lists = sub_lists(geom, ind, num)
+    for lst in lists:
+        textures_accumulate(geom, woi, clt, lst, params, rbga=rbga, rbga_stride=rbga_stride,
+                            ports=ports)
+    return t, woi, num, lists, rbga
+
+
+def debayer_shot(src, rb_mode, min_shot=10.0, scale_shot=3.0):
+    src = np.ascontiguousarray(src, np.float64)
+    dst = np.zeros((16, 16))
+    lib().oracle_debayer_shot_f64(ctypes.c_int(rb_mode), ctypes.c_float(min_shot),
+                                  ctypes.c_float(scale_shot), src.ctypes.data_as(_VP),
+                                  dst.ctypes.data_as(_VP))
+    return dst
+
+
+def combine_pixel(D, M, r, c, colors=3, diff_sigma=1.5, weights=(0.294118, 0.117647, 0.588235),
+                  dust_remove=1, keep_weights=1, ports=PORTS):
+    D = np.zeros((4, 3)) + np.asarray(D, np.float64)
+    M = np.zeros((4, 3)) + np.asarray(M, np.float64)
+    out = np.zeros(12)
+    po = np.ascontiguousarray(ports, np.float32)
+    f = ctypes.c_float
+    lib().oracle_combine_pixel_f64(ctypes.c_int(colors), f(diff_sigma), f(weights[0]), f(weights[1]),
+                                   f(weights[2]), ctypes.c_int(dust_remove), ctypes.c_int(keep_weights),
+                                   po.ctypes.data_as(_VP), D.ctypes.data_as(_VP), M.ctypes.data_as(_VP),
+                                   ctypes.c_int(r), ctypes.c_int(c), out.ctypes.data_as(_VP))
+    return out
+
+
+def corr_combine(corrs, radius, pair_mask=0x3F):
+    """corrs: [tiles * 6][>= (2r+1)^2] rows of a tile-major, pair-minor list
+    -> [tiles][(2r+1)^2] combined tiles."""
+    corrs = np.ascontiguousarray(corrs, np.float64)
+    n = 2 * radius + 1
+    nt = corrs.shape[0] // 6
+    out = np.zeros((nt, n * n))
+    lib().oracle_corr_combine_f64(corrs.ctypes.data_as(_VP), ctypes.c_size_t(corrs.shape[1]),
+                                  ctypes.c_int(nt), ctypes.c_int(pair_mask), ctypes.c_int(radius),
+                                  out.ctypes.data_as(_VP))
+    return out
+
+
+def corr_disparity(tiles, radius, mode=0):
+    """tiles: [n][(2r+1)^2] -> [n][2] (disparity, strength); mode 0 parabola,
+    1 centroid."""
+    tiles = np.ascontiguousarray(tiles, np.float64)
+    out = np.zeros((tiles.shape[0], 2))
+    lib().oracle_corr_disparity_f64(tiles.ctypes.data_as(_VP), ctypes.c_size_t(tiles.shape[1]),
+                                    ctypes.c_int(tiles.shape[0]), ctypes.c_int(radius),
+                                    ctypes.c_int(mode), out.ctypes.data_as(_VP))
+    return out

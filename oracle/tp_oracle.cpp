@@ -23,7 +23,10 @@
  *   - correlation list entries with pair > 5 or a tile outside the grid,
  *     and tasks whose (tx, ty) is outside the grid, are skipped;
  *   - imclt_rbg debug writes (tile (0,0) := 100, tile (161,111) doubled) are
- *     not reproduced.
+ *     not reproduced;
+ *   - texture rms: a negative mean-square residue (equal values across
+ *     cameras) is clamped to 0 before the square root (the reference's
+ *     sqrtf returns NaN there).
  * The 8-point DCT-IV/DST-IV/DCT-II/DST-II butterflies of the reference are
  * replaced by their matrices (the reference's butterflies equal the
  * textbook orthonormal DCT-IV/DST-IV, and DCT-II/DST-II with the first /
@@ -588,6 +591,336 @@ void imclt_rbg(const Geo &g, const T *clt, T *rbg, int color, int v_off, int h_o
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Textures: textures_accumulate :2207-2585 with debayer_shot           */
+/* :4019-4144, debayer :4146-4199 and tile_combine_rgba :4202-4668       */
+/* ------------------------------------------------------------------ */
+struct TexParams {
+    int colors;
+    float min_shot, scale_shot, diff_sigma, diff_threshold, min_agree;
+    float weights[3];
+    int dust_remove;
+    int keep_weights;
+};
+
+inline int sym16(int i) { return i < 8 ? i : 15 - i; }
+
+/* debayer_shot: un-window (1/w^2), 3x3 debayer (interior 14x14, border
+ * copied), shot-noise companding, re-window.  The reference's debayer
+ * arithmetic uses double literals, so it is evaluated in double. */
+template <typename T>
+void debayer_shot(int rb_mode, float min_shot, float scale_shot, const T src[16][16], T dst[16][16]) {
+    const Tables<T> &tb = tables<T>();
+    T tmp[16][16];
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) tmp[r][c] = src[r][c] * tb.hwsqi[sym16(c)] * tb.hwsqi[sym16(r)];
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) {
+            if (r == 0 || r == 15 || c == 0 || c == 15) {
+                dst[r][c] = tmp[r][c];
+                continue;
+            }
+            double cross = (double)tmp[r - 1][c] + (double)tmp[r + 1][c] + (double)tmp[r][c - 1] + (double)tmp[r][c + 1];
+            if (rb_mode) {
+                double diag = (double)tmp[r - 1][c - 1] + (double)tmp[r - 1][c + 1] +
+                              (double)tmp[r + 1][c - 1] + (double)tmp[r + 1][c + 1];
+                dst[r][c] = (T)(0.0625 * diag + 0.125 * cross + 0.25 * (double)tmp[r][c]);
+            } else {
+                dst[r][c] = (T)(0.125 * cross + 0.5 * (double)tmp[r][c]);
+            }
+        }
+    if (scale_shot > 0.0f) {
+        T k = (T)1 / std::sqrt((T)min_shot);
+        for (int r = 0; r < 16; r++)
+            for (int c = 0; c < 16; c++) {
+                T d = dst[r][c];
+                dst[r][c] = (T)scale_shot * ((d > (T)min_shot) ? std::sqrt(d) : k * d);
+            }
+    }
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) dst[r][c] *= tb.hwsq[sym16(c)] * tb.hwsq[sym16(r)];
+}
+
+/* tile_combine_rgba for one pixel.  D[cam][color]: debayered, M: not
+ * debayered.  out: colors values, alpha, 4 port weights, colors+1 rms. */
+template <typename T>
+void combine_pixel(const TexParams &p, const float port_offsets[4][2], const T D[4][3],
+                   const T M[4][3], int r, int c, T *out) {
+    const Tables<T> &tb = tables<T>();
+    const int colors = p.colors;
+    /* pairs in the reference's (i < j) loop order */
+    int pp[6][2];
+    T dist2r[6];
+    const T ksigma = (T)(1.0 / (2.0 * (double)p.diff_sigma * (double)p.diff_sigma));
+    int n = 0;
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++) {
+            T dx = (T)port_offsets[j][0] - (T)port_offsets[i][0];
+            T dy = (T)port_offsets[j][1] - (T)port_offsets[i][1];
+            pp[n][0] = i;
+            pp[n][1] = j;
+            dist2r[n++] = ksigma / (dx * dx + dy * dy);
+        }
+    T *rgba = out;
+    T *alpha = out + colors;
+    T *pw = out + colors + 1;
+    T *crms = out + colors + 5;
+    if (p.keep_weights) {
+        T sw = 0;
+        for (int col = 0; col < colors; col++) {
+            T s0 = 0, s1 = 0, s2 = 0;
+            for (int cam = 0; cam < 4; cam++) {
+                s0 += 1;
+                s1 += D[cam][col];
+                s2 += D[cam][col] * D[cam][col];
+            }
+            T mse = (s0 * s2 - s1 * s1) / (s0 * s0);
+            mse = std::max(mse, (T)0);  /* rounding residue of equal values */
+            crms[col] = std::sqrt(mse);
+            sw += (T)p.weights[col] * mse;
+        }
+        crms[colors] = std::sqrt(sw);
+    }
+    const T wnd2 = tb.hwsq[sym16(r)] * tb.hwsq[sym16(c)];
+    const T wnd2_inv = (T)1 / wnd2;
+    for (int cam = 0; cam < 4; cam++) pw[cam] = 0;
+    for (int ip = 0; ip < 6; ip++) {
+        T d = 0;
+        for (int col = 0; col < colors; col++) {
+            T dc = (D[pp[ip][0]][col] - D[pp[ip][1]][col]) * wnd2_inv;
+            d += (T)p.weights[col] * dc * dc;
+        }
+        d = std::exp(-dist2r[ip] * d) + (T)TP_FAT_ZERO_WEIGHT;
+        pw[pp[ip][0]] += d;
+        pw[pp[ip][1]] += d;
+    }
+    int b1 = 0;
+    T best = pw[0];
+    for (int cam = 1; cam < 4; cam++)
+        if (pw[cam] > best) { b1 = cam; best = pw[cam]; }
+    int b2 = (b1 == 0) ? 1 : 0;
+    best = pw[b2];
+    for (int cam = b2 + 1; cam < 4; cam++)
+        if (cam != b1 && pw[cam] > best) { b2 = cam; best = pw[cam]; }
+    const T w1 = pw[b1] / (pw[b1] + pw[b2]);
+    const T w2 = (T)1 - w1;
+    for (int col = 0; col < colors; col++) rgba[col] = w1 * D[b1][col] + w2 * D[b2][col];
+    for (int cam = 0; cam < 4; cam++) {
+        T d2 = 0;
+        for (int col = 0; col < colors; col++) {
+            T dc = (D[cam][col] - rgba[col]) * wnd2_inv;
+            d2 += (T)p.weights[col] * dc * dc;
+        }
+        pw[cam] = std::exp(-ksigma * d2) + (T)TP_FAT_ZERO_WEIGHT;
+    }
+    if (p.dust_remove) {
+        int worst = 0;
+        T wv = pw[0];
+        for (int cam = 1; cam < 4; cam++)
+            if (pw[cam] < wv) { worst = cam; wv = pw[cam]; }
+        T avg = -wv;
+        for (int cam = 0; cam < 4; cam++) avg += pw[cam];
+        avg /= (T)3;
+        T scale = (T)1 + wv * (avg - wv) / (avg * avg * (T)3);
+        for (int cam = 0; cam < 4; cam++)
+            if (cam != worst) pw[cam] *= scale;
+        pw[worst] *= wv / avg;
+    }
+    /* output color: port-weighted average of the non-debayered tiles */
+    T k = 0;
+    for (int cam = 0; cam < 4; cam++) k += pw[cam];
+    k = (T)1 / k;
+    for (int col = 0; col < colors; col++) {
+        T acc = 0;
+        for (int cam = 0; cam < 4; cam++) acc += k * pw[cam] * M[cam][col];
+        rgba[col] = acc;
+    }
+    T a = 0;
+    for (int cam = 0; cam < 4; cam++) a += pw[cam];
+    *alpha = wnd2 * a / (T)4;
+}
+
+/* one texture tile: slices [colors + 1 (+ 4 + colors + 1)][16][16] */
+template <typename T>
+void texture_tile(const Geo &g, const T *const *clt, int tile, const TexParams &p,
+                  const float port_offsets[4][2], T *slices) {
+    const int colors = p.colors;
+    T M[4][3][16][16], D[4][3][16][16];
+    T lb[64];
+    for (int cam = 0; cam < 4; cam++)
+        for (int col = 0; col < colors; col++) {
+            const T *src = clt[cam] + ((size_t)tile * TP_NUM_COLORS + col) * 256;
+            const T *lpf = lpf_table<T>(colors > 1 ? col : 3, lb);
+            T q[256];
+            for (int i = 0; i < 256; i++) q[i] = src[i] * lpf[i & 63];
+            imclt_tile<T>(q, M[cam][col], 0);
+            if (colors > 1)
+                debayer_shot<T>(col < 2, p.min_shot, p.scale_shot, M[cam][col], D[cam][col]);
+            else
+                std::memcpy(D[cam][col], M[cam][col], sizeof(D[cam][col]));
+        }
+    const int nsl = colors + 1 + 4 + colors + 1;
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) {
+            T dp[4][3], mp[4][3], out[12];
+            for (int cam = 0; cam < 4; cam++)
+                for (int col = 0; col < colors; col++) {
+                    dp[cam][col] = D[cam][col][r][c];
+                    mp[cam][col] = M[cam][col][r][c];
+                }
+            for (int i = 0; i < 12; i++) out[i] = 0;
+            combine_pixel<T>(p, port_offsets, dp, mp, r, c, out);
+            for (int s = 0; s < nsl; s++) slices[s * 256 + r * 16 + c] = out[s];
+        }
+}
+
+/* textures_accumulate over a list: non-overlapping tiles and/or the
+ * alpha-faded overlap-add into the woi-relative RGBA image. */
+template <typename T>
+void textures_accumulate(const Geo &g, const int *woi, const T *const *clt, size_t n,
+                         const int *idx, const float port_offsets[4][2], const TexParams &p,
+                         size_t rbga_stride, T *rbga, size_t tex_stride, T *tiles) {
+    const int colors = p.colors;
+    const int nsl_all = colors + 1 + 4 + colors + 1;
+    const int nsl_tile = p.keep_weights ? nsl_all : colors + 1;
+    /* tiles of one list never overlap in the image only within a parity
+     * pass; accumulate serially to keep the order-independent sum exact */
+    std::vector<T> sl(nsl_all * 256);
+    for (size_t e = 0; e < n; e++) {
+        const int code = idx[e];
+        if (!(code & (1 << TP_LIST_TEXTURE_BIT))) continue;
+        const int tile = code >> TP_CORR_NTILE_SHIFT;
+        if (tile < 0 || tile >= g.TX * g.TY) continue;
+        texture_tile<T>(g, clt, tile, p, port_offsets, sl.data());
+        if (tiles && tex_stride) {
+            for (int s = 0; s < nsl_tile; s++)
+                for (int i = 0; i < 256; i++) tiles[e * tex_stride + s * 256 + i] = sl[s * 256 + i];
+        }
+        const int tcode = code & TP_TASK_TEXTURE_BITS;
+        if (!tcode || !rbga || !rbga_stride) continue;
+        if (tcode != TP_TASK_TEXTURE_BITS)
+            for (int s = 0; s <= colors; s++)
+                for (int i = 0; i < 256; i++) sl[s * 256 + i] *= tables<T>().afade[tcode][i];
+        const size_t slice_stride = rbga_stride * (size_t)(woi[3] + 1) * 8;
+        const int ty = tile / g.TX, tx = tile % g.TX;
+        const int x0 = (tx - woi[0]) * 8, y0 = (ty - woi[1]) * 8;
+        for (int s = 0; s <= colors; s++)
+            for (int r = 0; r < 16; r++)
+                for (int c = 0; c < 16; c++)
+                    rbga[s * slice_stride + (size_t)(y0 + r) * rbga_stride + x0 + c] += sl[s * 256 + r * 16 + c];
+    }
+}
+
+/* prepare_texture_list :1643-1713 (with the child kernels :1716-1854),
+ * sequentially.  Mutates the texture bits of tasks (N, E, S, W neighbours
+ * present), fills the 4 parity sub-lists (inner tiles from the start,
+ * border tiles from the end), num_texture_tiles[8] and woi (x, y, width,
+ * height). */
+void prepare_texture_list(const Geo &g, tp_task *tasks, int num_tiles, int *indices, int *num_tex,
+                          int *woi, int width, int height) {
+    for (int row = 0; row < height; row++)
+        for (int col = 0; col < width && col < g.TX; col++) indices[col + row * g.TX] = 0;
+    std::vector<int> occ(g.TX * g.TY, 0);
+    for (int t = 0; t < num_tiles; t++)
+        if (tasks[t].task & TP_TASK_TEXTURE_BITS) {
+            int x = tasks[t].txy & 0xffff, y = tasks[t].txy >> 16;
+            occ[x + y * g.TX] = 1;
+        }
+    woi[0] = g.TX; woi[1] = g.TY; woi[2] = 0; woi[3] = 0;
+    for (int t = 0; t < num_tiles; t++) {
+        int task = tasks[t].task;
+        if (!(task & TP_TASK_TEXTURE_BITS)) continue;
+        int x = tasks[t].txy & 0xffff, y = tasks[t].txy >> 16;
+        woi[0] = std::min(woi[0], x); woi[1] = std::min(woi[1], y);
+        woi[2] = std::max(woi[2], x); woi[3] = std::max(woi[3], y);
+        int d = 0;
+        if (y > 0 && occ[x + (y - 1) * g.TX]) d |= 1;
+        if (x < g.TX - 1 && occ[x + 1 + y * g.TX]) d |= 2;
+        if (y < g.TY - 1 && occ[x + (y + 1) * g.TX]) d |= 4;
+        if (x > 0 && occ[x - 1 + y * g.TX]) d |= 8;
+        tasks[t].task = ((task ^ d) & TP_TASK_TEXTURE_BITS) ^ task;
+    }
+    for (int i = 0; i < 8; i++) num_tex[i] = 0;
+    const int quarter = g.TX * (g.TYA >> 2);
+    for (int t = 0; t < num_tiles; t++) {
+        int task = tasks[t].task & TP_TASK_TEXTURE_BITS;
+        if (!task) continue;
+        int x = tasks[t].txy & 0xffff, y = tasks[t].txy >> 16;
+        int border = (x == woi[0]) || (y == woi[1]) || (x == g.TX - 1) || (y == woi[3]);
+        int head = 0, slot = 0;
+        if (x & 1) { head += quarter; slot += 2; }
+        if (y & 1) { head += 2 * quarter; slot += 4; }
+        if (border) { head += quarter - 1; slot += 1; }
+        int pos = num_tex[slot]++;
+        indices[head + (border ? -pos : pos)] = task | ((x + y * g.TX) << TP_CORR_NTILE_SHIFT) | (1 << TP_LIST_TEXTURE_BIT);
+    }
+    woi[2] += 1 - woi[0];
+    woi[3] += 1 - woi[1];
+}
+
+/* ------------------------------------------------------------------ */
+/* Pair combination and sub-pixel disparity (not in the reference: its  */
+/* Java host does this).  Each pair's tile is mapped onto the horizontal */
+/* disparity axis (pairs 0,1 horizontal: as is; 2,3 vertical: transposed;*/
+/* 4 (0,3) diagonal: C[x + y][x - y]; 5 (2,1) anti-diagonal:             */
+/* C[y - x][x + y], offsets from the centre) and the mapped tiles are    */
+/* summed.  A residual disparity dd then peaks at (0, +dd).              */
+/* ------------------------------------------------------------------ */
+inline int pair_src(int pair, int r, int cy, int cx, int &sy, int &sx) {
+    switch (pair) {
+    case 0: case 1: sy = cy; sx = cx; break;
+    case 2: case 3: sy = cx; sx = cy; break;
+    case 4: sy = cx + cy; sx = cx - cy; break;
+    default: sy = cy - cx; sx = cx + cy; break;
+    }
+    return (sy >= -r && sy <= r && sx >= -r && sx <= r);
+}
+
+template <typename T>
+void corr_combine(const T *const *rows, const int *pairs, int npairs, int r, T *out) {
+    const int n = 2 * r + 1;
+    for (int y = -r; y <= r; y++)
+        for (int x = -r; x <= r; x++) {
+            T s = 0;
+            for (int k = 0; k < npairs; k++) {
+                int sy, sx;
+                if (pair_src(pairs[k], r, y, x, sy, sx)) s += rows[k][(sy + r) * n + sx + r];
+            }
+            out[(y + r) * n + x + r] = s;
+        }
+}
+
+/* Disparity from the centre row of a combined tile: integer argmax, then
+ * parabola through the 3 samples (mode 0) or centroid of the positive part
+ * of the samples within +-2 of the argmax (mode 1).  out: disparity,
+ * strength (peak value). */
+template <typename T>
+void corr_disparity(const T *tile, int r, int mode, T *out) {
+    const int n = 2 * r + 1;
+    const T *row = tile + r * n;
+    int best = 0;
+    for (int x = 1; x < n; x++)
+        if (row[x] > row[best]) best = x;
+    T d = (T)(best - r);
+    if (mode == 0) {
+        if (best > 0 && best < n - 1) {
+            T a = row[best - 1], b = row[best], c = row[best + 1];
+            T den = a - 2 * b + c;
+            if (den < 0) d += (T)0.5 * (a - c) / den;
+        }
+    } else {
+        T sw = 0, sx = 0;
+        for (int x = std::max(0, best - 2); x <= std::min(n - 1, best + 2); x++) {
+            T w = std::max(row[x], (T)0);
+            sw += w;
+            sx += w * (T)(x - r);
+        }
+        if (sw > 0) d = sx / sw;
+    }
+    out[0] = d;
+    out[1] = row[best];
+}
+
 }  // namespace
 
 /* ====================================================================== */
@@ -705,6 +1038,83 @@ void oracle_imclt_rbg_f32(const tp_geometry *geom, const float *clt, float *rbg,
 void oracle_imclt_rbg_f64(const tp_geometry *geom, const double *clt, double *rbg, int color,
                           int v_off, int h_off, size_t dstride) {
     imclt_rbg<double>(geo_of(geom), clt, rbg, color, v_off, h_off, dstride);
+}
+
+
+static TexParams tex_params(int colors, float min_shot, float scale_shot, float diff_sigma,
+                            float diff_threshold, float min_agree, float w0, float w1, float w2,
+                            int dust_remove, int keep_weights) {
+    TexParams p;
+    p.colors = colors; p.min_shot = min_shot; p.scale_shot = scale_shot;
+    p.diff_sigma = diff_sigma; p.diff_threshold = diff_threshold; p.min_agree = min_agree;
+    p.weights[0] = w0; p.weights[1] = w1; p.weights[2] = w2;
+    p.dust_remove = dust_remove; p.keep_weights = keep_weights;
+    return p;
+}
+
+/* textures_accumulate (host arrays; clt precision selects the variant) */
+void oracle_textures_accumulate_f64(const tp_geometry *geom, const int *woi, const double *const *clt,
+                                    size_t n, const int *idx, const float *port_offsets, int colors,
+                                    float min_shot, float scale_shot, float diff_sigma,
+                                    float diff_threshold, float min_agree, float w0, float w1,
+                                    float w2, int dust_remove, int keep_weights, size_t rbga_stride,
+                                    double *rbga, size_t tex_stride, double *tiles) {
+    TexParams p = tex_params(colors, min_shot, scale_shot, diff_sigma, diff_threshold, min_agree,
+                             w0, w1, w2, dust_remove, keep_weights);
+    textures_accumulate<double>(geo_of(geom), woi, clt, n, idx, (const float(*)[2])port_offsets, p,
+                                rbga_stride, rbga, tex_stride, tiles);
+}
+void oracle_textures_accumulate_f32(const tp_geometry *geom, const int *woi, const float *const *clt,
+                                    size_t n, const int *idx, const float *port_offsets, int colors,
+                                    float min_shot, float scale_shot, float diff_sigma,
+                                    float diff_threshold, float min_agree, float w0, float w1,
+                                    float w2, int dust_remove, int keep_weights, size_t rbga_stride,
+                                    float *rbga, size_t tex_stride, float *tiles) {
+    TexParams p = tex_params(colors, min_shot, scale_shot, diff_sigma, diff_threshold, min_agree,
+                             w0, w1, w2, dust_remove, keep_weights);
+    textures_accumulate<float>(geo_of(geom), woi, clt, n, idx, (const float(*)[2])port_offsets, p,
+                               rbga_stride, rbga, tex_stride, tiles);
+}
+
+void oracle_prepare_texture_list(const tp_geometry *geom, tp_task *tasks, int num_tiles,
+                                 int *indices, int *num_tex, int *woi, int width, int height) {
+    prepare_texture_list(geo_of(geom), tasks, num_tiles, indices, num_tex, woi, width, height);
+}
+
+/* single debayer_shot / combine for unit tests */
+void oracle_debayer_shot_f64(int rb_mode, float min_shot, float scale_shot, const double *src,
+                             double *dst) {
+    debayer_shot<double>(rb_mode, min_shot, scale_shot, (const double(*)[16])src, (double(*)[16])dst);
+}
+void oracle_combine_pixel_f64(int colors, float diff_sigma, float w0, float w1, float w2,
+                              int dust_remove, int keep_weights, const float *port_offsets,
+                              const double *D, const double *M, int r, int c, double *out) {
+    TexParams p = tex_params(colors, 10.f, 3.f, diff_sigma, 10.f, 3.f, w0, w1, w2, dust_remove,
+                             keep_weights);
+    combine_pixel<double>(p, (const float(*)[2])port_offsets, (const double(*)[3])D,
+                          (const double(*)[3])M, r, c, out);
+}
+
+/* combine: corrs rows of the 6 pairs of each tile (row index = tile * 6 +
+ * pair in a tile-major list) -> out[tile][(2r+1)^2] */
+void oracle_corr_combine_f64(const double *corrs, size_t stride, int ntiles, int pair_mask,
+                             int radius, double *out) {
+    const int n = 2 * radius + 1;
+    for (int t = 0; t < ntiles; t++) {
+        const double *rows[6];
+        int pairs[6], np = 0;
+        for (int p = 0; p < 6; p++)
+            if ((pair_mask >> p) & 1) {
+                rows[np] = corrs + ((size_t)t * 6 + p) * stride;
+                pairs[np++] = p;
+            }
+        corr_combine<double>(rows, pairs, np, radius, out + (size_t)t * n * n);
+    }
+}
+
+void oracle_corr_disparity_f64(const double *tiles, size_t stride, int ntiles, int radius, int mode,
+                               double *out) {
+    for (int t = 0; t < ntiles; t++) corr_disparity<double>(tiles + (size_t)t * stride, radius, mode, out + 2 * t);
 }
 
 }  // extern "C"

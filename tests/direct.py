@@ -88,3 +88,26 @@ def forward_direct(img, color, itlx, itly, sh, sv, kernel=None):
     if kernel is not None:
         q = bicomplex_mul(q, np.asarray(kernel, np.float64).reshape(4, 8, 8))
     return bicomplex_mul(q, phasor(sh, sv))
+
+
+def smooth_scene(seed, n=24, fmax=0.12):
+    """Band-limited gray scene f(x, y) = sum a cos(2 pi (u x + v y) + phi)."""
+    rng = np.random.default_rng(seed)
+    u, v = rng.uniform(-fmax, fmax, (2, n))
+    a = rng.uniform(0.5, 1.0, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+
+    def f(x, y):
+        x = np.asarray(x, np.float64)[..., None]
+        y = np.asarray(y, np.float64)[..., None]
+        return (a * np.cos(2 * np.pi * (u * x + v * y) + ph)).sum(-1)
+    return f
+
+
+def render_cameras(geom, disparity, seed=0, fmax=0.3, ports=((-0.5, -0.5), (0.5, -0.5), (-0.5, 0.5), (0.5, 0.5))):
+    """4 gray cameras of a band-limited scene; camera i sees the content
+    displaced by disparity * port_i (disparity: scalar or [H][W] map)."""
+    f = smooth_scene(seed, n=64, fmax=fmax)
+    y, x = np.mgrid[0:geom.img_height, 0:geom.img_width].astype(np.float64)
+    d = np.broadcast_to(np.asarray(disparity, np.float64), y.shape)
+    return np.stack([f(x - d * px, y - d * py) for px, py in ports]).astype(np.float32)

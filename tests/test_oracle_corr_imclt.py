@@ -6,7 +6,7 @@ import scipy.fft as sf
 
 from tile_processor_gpu_amd.geometry import Geometry, full_tasks, corr_indices, PAIRS
 from tile_processor_gpu_amd import synth
-from direct import bicomplex_mul, bicomplex_conj, lapped_basis, bayer_mask
+from direct import bicomplex_mul, bicomplex_conj, lapped_basis, bayer_mask, smooth_scene
 
 LPF_RB, LPF_C = 4, 5
 
@@ -79,20 +79,6 @@ def test_mono_correlation_uses_color0_only(orc):
     A = [clt[0][9 * 256:10 * 256].reshape(4, 8, 8)]
     B = [clt[3][9 * 256:10 * 256].reshape(4, 8, 8)]
     np.testing.assert_allclose(out[0], corr_direct(A, B, (0.75,), 0.125, orc.table(3), 7).ravel(), atol=1e-13)
-
-
-def smooth_scene(seed, n=24, fmax=0.12):
-    """Band-limited gray scene f(x, y) = sum a cos(2 pi (u x + v y) + phi)."""
-    rng = np.random.default_rng(seed)
-    u, v = rng.uniform(-fmax, fmax, (2, n))
-    a = rng.uniform(0.5, 1.0, n)
-    ph = rng.uniform(0, 2 * np.pi, n)
-
-    def f(x, y):
-        x = np.asarray(x, np.float64)[..., None]
-        y = np.asarray(y, np.float64)[..., None]
-        return (a * np.cos(2 * np.pi * (u * x + v * y) + ph)).sum(-1)
-    return f
 
 
 def _shifted_pair(dx, dy, seed=0, size=128, fmax=0.12):

@@ -25,6 +25,11 @@ PROTOTYPES = {
     "tp_correlate2D": [_G, _P, _I, _F, _F, _F, _F, _SZ, _P, _SZ, _I, _P, _P],
     "tp_imclt_rbg": [_G, _P, _P, _I, _I, _I, _SZ, _P],
     "tp_imclt_rbg_all": [_G, _P, _P, _SZ, _P],
+    "tp_textures_accumulate": [_G, _P, _P, _SZ, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F, _F, _F,
+                               _I, _I, _SZ, _P, _SZ, _P, _P],
+    "tp_prepare_texture_list": [_G, _P, _I, _P, _P, _P, _I, _I, _P],
+    "tp_generate_RBGA": [_G, _P, _I, _P, _P, _P, _I, _I, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F,
+                         _F, _F, _I, _I, _SZ, _P, _P],
 }
 _RESTYPES = {"tp_last_error": ctypes.c_char_p, "tp_version": ctypes.c_char_p}
 

@@ -1,0 +1,339 @@
+// k_texture.hpp - texture generation (plain FP32).
+//
+// Replaces textures_accumulate (src/TileProcessor.cuh:2207-2585) with
+// imclt8threads (:3847-4017), debayer_shot (:4019-4144), debayer
+// (:4146-4199) and tile_combine_rgba (:4202-4668), and the texture-list
+// kernels of generate_RBGA / prepare_texture_list (:1446-1854).
+//
+// One workgroup (4 wavefronts) per texture tile:
+//   1. the 4 cameras x colors units run the inverse CLT of the low-passed
+//      CLT (8 lanes per unit), results M[cam][color][16][16] in LDS;
+//   2. thread = pixel: un-window, 3x3 debayer, shot-noise companding,
+//      re-window -> D in LDS;
+//   3. thread = pixel: 4-camera consensus (pairwise port weights, best-pair
+//      blend, per-port weights, dust removal), colour from M, alpha;
+//      non-overlapping tile slices and/or the alpha-faded overlap-add into
+//      the RGBA image.
+// The list kernels replace CUDA dynamic parallelism: the host enqueues them
+// in order and later grids read their sizes from device memory, so the
+// whole generate_RBGA sequence is stream-ordered and graph-capturable.
+#pragma once
+
+#include "tp_common.hpp"
+#include "k_imclt.hpp"
+
+namespace tp {
+
+struct TexArgs {
+    Geo g;
+    const int *woi;
+    const float *const *clt;
+    int n;              // list length, or the cap of *num_ptr
+    const int *num_ptr; // device count (generate_RBGA passes)
+    const int *idx;
+    int border_quarter; // > 0: list is the last n entries of a quarter of this size
+    const float *ports; // [4][2]
+    int colors;
+    float min_shot, scale_shot, diff_sigma, diff_threshold, min_agree;
+    float w[3];
+    int dust_remove, keep_weights;
+    size_t rbga_stride;
+    float *rbga;
+    size_t tex_stride;
+    float *tiles;
+};
+
+__device__ __forceinline__ float hw_sq(int i) {  // HWINDOW_SQ, runtime index
+    float h = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) h = (i == t) ? kTab.hw[t] * kTab.hw[t] : h;
+    return h;
+}
+
+// alphaFade[m][i][j] (:492-1005): fade(t) = (1 - cos(pi (t + 1/2)/16)) / 2
+__device__ __forceinline__ float alpha_fade(int m, int i, int j) {
+    auto fade = [](int t) { return 0.5f * (1.0f - cospif((t + 0.5f) * 0.0625f)); };
+    switch (m) {
+    case 0: return 0.f;
+    case 1: return fade(15 - i);
+    case 2: return fade(j);
+    case 4: return fade(i);
+    case 8: return fade(15 - j);
+    case 3: return (j > 15 - i) ? fade(j) : fade(15 - i);
+    case 6: return (j > i) ? fade(j) : fade(i);
+    case 9: return (j > i) ? fade(15 - i) : fade(15 - j);
+    case 12: return (i > 15 - j) ? fade(i) : fade(15 - j);
+    default: return 1.f;
+    }
+}
+
+template <int COLORS>
+__global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
+    __shared__ float M[12 * 256];  // [cam * colors + color][256]
+    __shared__ __align__(16) union U {
+        float scratch[4][8][4][72];  // phase 1: imclt transposes
+        struct {
+            float tmp[12 * 256];
+            float D[12 * 256];
+        } p2;
+    } u;
+    const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
+    const int code = list[e];
+    if (!(code & (1 << TP_LIST_TEXTURE_BIT))) return;
+    const int tile = code >> TP_CORR_NTILE_SHIFT;
+    if (tile < 0 || tile >= a.g.TX * a.g.TY) return;
+    constexpr int colors = COLORS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = wave * 8 + (lane >> 3), li = lane & 7;
+
+    // phase 1: M = imclt(CLT * lpf)
+    if (slot < 4 * colors) {
+        const int cam = slot / colors, col = slot % colors;
+        const int lw = colors > 1 ? col : 3;
+        float q0[8], q1[8], q2[8], q3[8];
+        load_rows(a.clt[cam] + ((size_t)tile * TP_NUM_COLORS + col) * 256, li, q0, q1, q2, q3);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float l = c_lpf.v[lw][li * 8 + k];
+            q0[k] *= l; q1[k] *= l; q2[k] *= l; q3[k] *= l;
+        }
+        imclt_transform(q0, q1, q2, q3, &u.scratch[wave][lane >> 3][0][0], li);
+        float *m = M + slot * 256;
+        imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
+    }
+    __syncthreads();
+
+    const int t = threadIdx.x, r = t >> 4, c = t & 15;
+    const int rs = r < 8 ? r : 15 - r, cs = c < 8 ? c : 15 - c;
+    const float wsq = hw_sq(rs) * hw_sq(cs);
+    // phase 2: debayer_shot (colors > 1), else D = M
+    if (colors > 1) {
+        const float wsqi_r = 1.0f / hw_sq(rs), wsqi_c = 1.0f / hw_sq(cs);
+        for (int s = 0; s < 4 * colors; s++) u.p2.tmp[s * 256 + t] = M[s * 256 + t] * wsqi_c * wsqi_r;
+        __syncthreads();
+        const float k = 1.0f / sqrtf(a.min_shot);
+        for (int s = 0; s < 4 * colors; s++) {
+            const float *x = u.p2.tmp + s * 256;
+            const int col = s % colors;
+            float d;
+            if (r == 0 || r == 15 || c == 0 || c == 15) {
+                d = x[t];
+            } else {
+                const float cross = (x[t - 16] + x[t + 16]) + (x[t - 1] + x[t + 1]);
+                if (col < 2) {
+                    const float diag = (x[t - 17] + x[t - 15]) + (x[t + 15] + x[t + 17]);
+                    d = fmaf(0.0625f, diag, fmaf(0.125f, cross, 0.25f * x[t]));
+                } else {
+                    d = fmaf(0.125f, cross, 0.5f * x[t]);
+                }
+            }
+            if (a.scale_shot > 0.0f) d = a.scale_shot * (d > a.min_shot ? sqrtf(d) : k * d);
+            u.p2.D[s * 256 + t] = d * wsq;
+        }
+    } else {
+        for (int s = 0; s < 4; s++) u.p2.D[s * 256 + t] = M[s * 256 + t];
+    }
+    __syncthreads();
+
+    // phase 3: consensus per pixel
+    float D[4][3], Mv[4][3];
+#pragma unroll
+    for (int cam = 0; cam < 4; cam++)
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            D[cam][col] = col < colors ? u.p2.D[(cam * colors + col) * 256 + t] : 0.f;
+            Mv[cam][col] = col < colors ? M[(cam * colors + col) * 256 + t] : 0.f;
+        }
+    const float ksigma = 1.0f / (2.0f * a.diff_sigma * a.diff_sigma);
+    const float wnd2_inv = 1.0f / wsq;
+    float out[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) out[i] = 0.f;
+    float *rgba = out, *pw = out + colors + 1, *crms = out + colors + 5;
+    if (a.keep_weights) {
+        float sw = 0.f;
+        for (int col = 0; col < colors; col++) {
+            float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+            for (int cam = 0; cam < 4; cam++) {
+                s1 += D[cam][col];
+                s2 = fmaf(D[cam][col], D[cam][col], s2);
+            }
+            const float mse = fmaxf(0.f, (4.f * s2 - s1 * s1) * 0.0625f);
+            crms[col] = sqrtf(mse);
+            sw = fmaf(a.w[col], mse, sw);
+        }
+        crms[colors] = sqrtf(sw);
+    }
+    float p[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = i + 1; j < 4; j++) {
+            const float dx = a.ports[2 * j] - a.ports[2 * i], dy = a.ports[2 * j + 1] - a.ports[2 * i + 1];
+            const float dist2r = ksigma / (dx * dx + dy * dy);
+            float d = 0.f;
+            for (int col = 0; col < colors; col++) {
+                const float dc = (D[i][col] - D[j][col]) * wnd2_inv;
+                d = fmaf(a.w[col] * dc, dc, d);
+            }
+            d = expf(-dist2r * d) + (float)TP_FAT_ZERO_WEIGHT;
+            p[i] += d;
+            p[j] += d;
+        }
+    int b1 = 0;
+    float best = p[0];
+#pragma unroll
+    for (int cam = 1; cam < 4; cam++)
+        if (p[cam] > best) { b1 = cam; best = p[cam]; }
+    int b2 = b1 == 0 ? 1 : 0;
+    best = p[b2];
+#pragma unroll
+    for (int cam = 1; cam < 4; cam++)
+        if (cam > (b1 == 0 ? 1 : 0) && cam != b1 && p[cam] > best) { b2 = cam; best = p[cam]; }
+    const float pb1 = b1 == 0 ? p[0] : b1 == 1 ? p[1] : b1 == 2 ? p[2] : p[3];
+    const float pb2 = b2 == 0 ? p[0] : b2 == 1 ? p[1] : b2 == 2 ? p[2] : p[3];
+    const float w1 = pb1 / (pb1 + pb2), w2 = 1.f - w1;
+    float blend[3];
+#pragma unroll
+    for (int col = 0; col < 3; col++) {
+        float d1 = 0.f, d2 = 0.f;
+#pragma unroll
+        for (int cam = 0; cam < 4; cam++) {
+            d1 = cam == b1 ? D[cam][col] : d1;
+            d2 = cam == b2 ? D[cam][col] : d2;
+        }
+        blend[col] = fmaf(w1, d1, w2 * d2);
+    }
+#pragma unroll
+    for (int cam = 0; cam < 4; cam++) {
+        float d2 = 0.f;
+        for (int col = 0; col < colors; col++) {
+            const float dc = (D[cam][col] - blend[col]) * wnd2_inv;
+            d2 = fmaf(a.w[col] * dc, dc, d2);
+        }
+        p[cam] = expf(-ksigma * d2) + (float)TP_FAT_ZERO_WEIGHT;
+    }
+    if (a.dust_remove) {
+        int worst = 0;
+        float wv = p[0];
+#pragma unroll
+        for (int cam = 1; cam < 4; cam++)
+            if (p[cam] < wv) { worst = cam; wv = p[cam]; }
+        const float avg = ((p[0] + p[1]) + (p[2] + p[3]) - wv) * (1.0f / 3.0f);
+        const float scale = 1.f + wv * (avg - wv) / (avg * avg * 3.f);
+#pragma unroll
+        for (int cam = 0; cam < 4; cam++) p[cam] *= (cam == worst) ? wv / avg : scale;
+    }
+    const float psum = (p[0] + p[1]) + (p[2] + p[3]);
+    const float kn = 1.f / psum;
+    for (int col = 0; col < colors; col++) {
+        float acc = 0.f;
+#pragma unroll
+        for (int cam = 0; cam < 4; cam++) acc = fmaf(kn * p[cam], Mv[cam][col], acc);
+        rgba[col] = acc;
+    }
+    out[colors] = wsq * psum * 0.25f;
+#pragma unroll
+    for (int cam = 0; cam < 4; cam++) pw[cam] = p[cam];
+
+    const int nsl_all = 2 * colors + 6;
+    if (a.tiles && a.tex_stride) {
+        const int nsl = a.keep_weights ? nsl_all : colors + 1;
+        float *dst = a.tiles + (size_t)e * a.tex_stride + t;
+        for (int s = 0; s < nsl; s++) dst[s * 256] = out[s];
+    }
+    const int tcode = code & TP_TASK_TEXTURE_BITS;
+    if (!tcode || !a.rbga || !a.rbga_stride) return;
+    const float f = tcode == TP_TASK_TEXTURE_BITS ? 1.f : alpha_fade(tcode, r, c);
+    const int ty = tile / a.g.TX, tx = tile - ty * a.g.TX;
+    const size_t slice = a.rbga_stride * (size_t)(a.woi[3] + 1) * 8;
+    float *dst = a.rbga + (size_t)((ty - a.woi[1]) * 8 + r) * a.rbga_stride + (tx - a.woi[0]) * 8 + c;
+    for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
+}
+
+// ---------------- texture list (generate_RBGA / prepare_texture_list) ----------------
+
+// zero the tile occupancy map (kept in the first tilesx*height entries of
+// the index buffer, like the reference) and reset woi / counters
+__global__ void k_tex_clear(int *indices, int width, int height, int TX, int TY, int *woi, int *num) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = min(width, TX);
+    if (i < w * height) indices[(i / w) * TX + (i % w)] = 0;
+    if (i == 0) {
+        woi[0] = TX; woi[1] = TY; woi[2] = 0; woi[3] = 0;
+    }
+    if (i < 8) num[i] = 0;
+}
+
+__device__ __forceinline__ bool task_xy(const tp_task &tk, int TX, int TY, int &x, int &y) {
+    x = tk.txy & 0xffff;
+    y = tk.txy >> 16;
+    return x < TX && y >= 0 && y < TY;
+}
+
+__global__ void k_tex_mark(const tp_task *tasks, int n, int *indices, int TX, int TY) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int x, y;
+    if (!(tasks[t].task & TP_TASK_TEXTURE_BITS) || !task_xy(tasks[t], TX, TY, x, y)) return;
+    indices[x + y * TX] = 1;
+}
+
+__global__ void k_tex_neighbors(tp_task *tasks, int n, const int *indices, int *woi, int TX, int TY) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int task = tasks[t].task;
+    int x, y;
+    if (!(task & TP_TASK_TEXTURE_BITS) || !task_xy(tasks[t], TX, TY, x, y)) return;
+    atomicMin(woi + 0, x);
+    atomicMin(woi + 1, y);
+    atomicMax(woi + 2, x);
+    atomicMax(woi + 3, y);
+    int d = 0;
+    if (y > 0 && indices[x + (y - 1) * TX]) d |= 1;
+    if (x < TX - 1 && indices[x + 1 + y * TX]) d |= 2;
+    if (y < TY - 1 && indices[x + (y + 1) * TX]) d |= 4;
+    if (x > 0 && indices[x - 1 + y * TX]) d |= 8;
+    tasks[t].task = ((task ^ d) & TP_TASK_TEXTURE_BITS) ^ task;
+}
+
+__global__ void k_tex_gen_list(const tp_task *tasks, int n, int *indices, int *num, const int *woi,
+                               int TX, int TY, int TYA) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int task = tasks[t].task & TP_TASK_TEXTURE_BITS;
+    int x, y;
+    if (!task || !task_xy(tasks[t], TX, TY, x, y)) return;
+    const int quarter = TX * (TYA >> 2);
+    const bool border = (x == woi[0]) || (y == woi[1]) || (x == TX - 1) || (y == woi[3]);
+    int head = 0, slot = 0;
+    if (x & 1) { head += quarter; slot += 2; }
+    if (y & 1) { head += 2 * quarter; slot += 4; }
+    if (border) { head += quarter - 1; slot += 1; }
+    const int pos = atomicAdd(num + slot, 1);
+    if (pos >= quarter) return;  // only possible with duplicate tiles in the task list
+    indices[head + (border ? -pos : pos)] = task | ((x + y * TX) << TP_CORR_NTILE_SHIFT) | (1 << TP_LIST_TEXTURE_BIT);
+}
+
+__global__ void k_tex_finish_woi(int *woi) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        woi[2] += 1 - woi[0];
+        woi[3] += 1 - woi[1];
+    }
+}
+
+// zero the RGBA image rows of the woi: (woi_w + 1) * 8 columns,
+// (woi_h + 1) * 8 rows per slice; grid covers the largest possible woi
+__global__ void k_tex_clear_rbga(float *rbga, size_t stride, const int *woi, int slices) {
+    const int wpx = (woi[2] + 1) * 8, hpx = (woi[3] + 1) * 8;
+    const int row = blockIdx.y;  // slice-major rows
+    if (row >= hpx * slices) return;
+    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < wpx; x += gridDim.x * blockDim.x)
+        rbga[(size_t)row * stride + x] = 0.f;
+}
+
+}  // namespace tp

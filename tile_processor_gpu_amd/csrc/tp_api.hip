@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <string>
 
 #include "tp_common.hpp"
 #include "k_convert.hpp"
 #include "k_correlate.hpp"
 #include "k_imclt.hpp"
+#include "k_texture.hpp"
 
 namespace {
 
@@ -165,6 +167,196 @@ int tp_imclt_rbg_all(const tp_geometry *geom, float **gpu_clt, float **gpu_rbg, 
     hipLaunchKernelGGL(tp::k_imclt_frame, grid, dim3(256), 0, as_stream(stream), g,
                        (const float *const *)gpu_clt, gpu_rbg, dstride);
     return check_launch("imclt_rbg_all");
+}
+
+}  // extern "C"
+
+namespace {
+
+int launch_textures(const tp::TexArgs &a, int grid, hipStream_t st) {
+    if (grid <= 0) return TP_OK;
+    if (a.colors == 3)
+        hipLaunchKernelGGL(tp::k_textures_accumulate<3>, dim3(grid), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(tp::k_textures_accumulate<1>, dim3(grid), dim3(256), 0, st, a);
+    return check_launch("textures_accumulate");
+}
+
+tp::TexArgs tex_args(const tp_geometry *geom, const int *woi, float **clt, const float *ports,
+                     int colors, float min_shot, float scale_shot, float diff_sigma,
+                     float diff_threshold, float min_agree, float w0, float w1, float w2,
+                     int dust_remove, int keep_weights) {
+    tp::TexArgs a;
+    a.g = tp::make_geo(*geom);
+    a.woi = woi;
+    a.clt = (const float *const *)clt;
+    a.n = 0;
+    a.num_ptr = nullptr;
+    a.idx = nullptr;
+    a.border_quarter = 0;
+    a.ports = ports;
+    a.colors = colors;
+    a.min_shot = min_shot;
+    a.scale_shot = scale_shot;
+    a.diff_sigma = diff_sigma;
+    a.diff_threshold = diff_threshold;
+    a.min_agree = min_agree;
+    a.w[0] = w0;
+    a.w[1] = w1;
+    a.w[2] = w2;
+    a.dust_remove = dust_remove;
+    a.keep_weights = keep_weights;
+    a.rbga_stride = 0;
+    a.rbga = nullptr;
+    a.tex_stride = 0;
+    a.tiles = nullptr;
+    return a;
+}
+
+int check_tex_params(int colors, float min_shot, float diff_sigma) {
+    if (colors != 1 && colors != 3) return fail(TP_EINVAL, "textures: colors must be 1 or 3%s", "");
+    if (!(min_shot > 0.f)) return fail(TP_EINVAL, "textures: min_shot must be > 0%s", "");
+    if (!(diff_sigma > 0.f)) return fail(TP_EINVAL, "textures: diff_sigma must be > 0%s", "");
+    return TP_OK;
+}
+
+// the list-building part of prepare_texture_list / generate_RBGA
+int enqueue_texture_list(const tp_geometry *geom, tp_task *tasks, int num_tiles, int *indices,
+                         int *num, int *woi, int width, int height, hipStream_t st) {
+    tp::Geo g = tp::make_geo(*geom);
+    const int nclr = std::max(8, std::min(width, g.TX) * height);
+    hipLaunchKernelGGL(tp::k_tex_clear, dim3((nclr + 255) / 256), dim3(256), 0, st, indices, width,
+                       height, g.TX, g.TY, woi, num);
+    int rc = check_launch("clear_texture_list");
+    if (rc) return rc;
+    if (num_tiles > 0) {
+        const int grid = (num_tiles + 255) / 256;
+        hipLaunchKernelGGL(tp::k_tex_mark, dim3(grid), dim3(256), 0, st, tasks, num_tiles, indices,
+                           g.TX, g.TY);
+        if ((rc = check_launch("mark_texture_tiles"))) return rc;
+        hipLaunchKernelGGL(tp::k_tex_neighbors, dim3(grid), dim3(256), 0, st, tasks, num_tiles,
+                           indices, woi, g.TX, g.TY);
+        if ((rc = check_launch("mark_texture_neighbor_tiles"))) return rc;
+        hipLaunchKernelGGL(tp::k_tex_gen_list, dim3(grid), dim3(256), 0, st, tasks, num_tiles,
+                           indices, num, woi, g.TX, g.TY, g.TYA);
+        if ((rc = check_launch("gen_texture_list"))) return rc;
+    }
+    hipLaunchKernelGGL(tp::k_tex_finish_woi, dim3(1), dim3(64), 0, st, woi);
+    return check_launch("texture woi");
+}
+
+int check_list_args(const tp_geometry *geom, const tp_task *tasks, int num_tiles, const int *indices,
+                    const int *num, const int *woi, int width, int height) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (!tasks || !indices || !num || !woi)
+        return fail(TP_EINVAL, "texture list: NULL pointer argument%s", "");
+    if (num_tiles < 0) return fail(TP_EINVAL, "texture list: num_tiles < 0%s", "");
+    if (width < 0 || height < 0 || height > geom->img_height / 8)
+        return fail(TP_EINVAL, "texture list: width/height out of range%s", "");
+    return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tp_textures_accumulate(const tp_geometry *geom, const int *woi, float **gpu_clt,
+                           size_t num_texture_tiles, const int *gpu_texture_indices,
+                           const float *gpu_port_offsets, int colors, int is_lwir, float min_shot,
+                           float scale_shot, float diff_sigma, float diff_threshold,
+                           float min_agree, float weight0, float weight1, float weight2,
+                           int dust_remove, int keep_weights, size_t texture_rbg_stride,
+                           float *gpu_texture_rbg, size_t texture_stride,
+                           float *gpu_texture_tiles, tp_stream_t stream) {
+    (void)is_lwir;  // unused by the reference kernel too
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if ((rc = check_tex_params(colors, min_shot, diff_sigma))) return rc;
+    if (!gpu_clt || !gpu_texture_indices || !gpu_port_offsets)
+        return fail(TP_EINVAL, "tp_textures_accumulate: NULL pointer argument%s", "");
+    const bool want_rbga = gpu_texture_rbg && texture_rbg_stride;
+    const bool want_tiles = gpu_texture_tiles && texture_stride;
+    if (want_rbga && !woi) return fail(TP_EINVAL, "tp_textures_accumulate: RGBA output needs woi%s", "");
+    const size_t nsl = (size_t)(keep_weights ? 2 * colors + 6 : colors + 1);
+    if (want_tiles && texture_stride < nsl * 256)
+        return fail(TP_EINVAL, "tp_textures_accumulate: texture_stride too small%s", "");
+    if (num_texture_tiles > (size_t)0x7fffffff) return fail(TP_EINVAL, "tp_textures_accumulate: too many tiles%s", "");
+    g_err.clear();
+    tp::TexArgs a = tex_args(geom, woi, gpu_clt, gpu_port_offsets, colors, min_shot, scale_shot,
+                             diff_sigma, diff_threshold, min_agree, weight0, weight1, weight2,
+                             dust_remove, keep_weights);
+    a.n = (int)num_texture_tiles;
+    a.idx = gpu_texture_indices;
+    if (want_rbga) {
+        a.rbga_stride = texture_rbg_stride;
+        a.rbga = gpu_texture_rbg;
+    }
+    if (want_tiles) {
+        a.tex_stride = texture_stride;
+        a.tiles = gpu_texture_tiles;
+    }
+    return launch_textures(a, (int)num_texture_tiles, as_stream(stream));
+}
+
+int tp_prepare_texture_list(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
+                            int *gpu_texture_indices, int *num_texture_tiles, int *woi, int width,
+                            int height, tp_stream_t stream) {
+    int rc = check_list_args(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_tiles, woi,
+                             width, height);
+    if (rc) return rc;
+    g_err.clear();
+    return enqueue_texture_list(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_tiles,
+                                woi, width, height, as_stream(stream));
+}
+
+int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
+                     int *gpu_texture_indices, int *num_texture_tiles, int *woi, int width,
+                     int height, float **gpu_clt, const float *gpu_port_offsets, int colors,
+                     int is_lwir, float min_shot, float scale_shot, float diff_sigma,
+                     float diff_threshold, float min_agree, float weight0, float weight1,
+                     float weight2, int dust_remove, int keep_weights, size_t texture_rbga_stride,
+                     float *gpu_texture_tiles, tp_stream_t stream) {
+    (void)is_lwir;
+    (void)keep_weights;  // the reference's passes run with keep_weights = 0
+    int rc = check_list_args(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_tiles, woi,
+                             width, height);
+    if (rc) return rc;
+    if ((rc = check_tex_params(colors, min_shot, diff_sigma))) return rc;
+    if (!gpu_clt || !gpu_port_offsets || !gpu_texture_tiles)
+        return fail(TP_EINVAL, "tp_generate_RBGA: NULL pointer argument%s", "");
+    if (texture_rbga_stride < (size_t)(geom->img_width / 8 + 1) * 8)
+        return fail(TP_EINVAL, "tp_generate_RBGA: texture_rbga_stride < (tilesx + 1) * 8%s", "");
+    g_err.clear();
+    hipStream_t st = as_stream(stream);
+    if ((rc = enqueue_texture_list(geom, gpu_tasks, num_tiles, gpu_texture_indices,
+                                   num_texture_tiles, woi, width, height, st)))
+        return rc;
+    tp::Geo g = tp::make_geo(*geom);
+    const int slices = colors + 1;
+    const int max_rows = (g.TY + 1) * 8 * slices;
+    hipLaunchKernelGGL(tp::k_tex_clear_rbga, dim3(((g.TX + 1) * 8 + 255) / 256, max_rows), dim3(256),
+                       0, st, gpu_texture_tiles, texture_rbga_stride, woi, slices);
+    if ((rc = check_launch("clear_texture_rbga"))) return rc;
+    tp::TexArgs a = tex_args(geom, woi, gpu_clt, gpu_port_offsets, colors, min_shot, scale_shot,
+                             diff_sigma, diff_threshold, min_agree, weight0, weight1, weight2,
+                             dust_remove, 0);
+    a.rbga_stride = texture_rbga_stride;
+    a.rbga = gpu_texture_tiles;
+    const int quarter = g.TX * (g.TYA >> 2);
+    for (int pass = 0; pass < 8; pass++) {  // :1562-1610
+        const int border = pass >> 2;
+        const int slot = ((pass & 3) << 1) + border;
+        tp::TexArgs p = a;
+        p.num_ptr = num_texture_tiles + slot;
+        // border lists grow down from the end of their quarter: the pass
+        // reads [end - n, end) with n known only on the device
+        p.idx = gpu_texture_indices + (pass & 3) * quarter;
+        p.border_quarter = border ? quarter : 0;
+        p.n = quarter;
+        if ((rc = launch_textures(p, quarter, st))) return rc;
+    }
+    return TP_OK;
 }
 
 }  // extern "C"

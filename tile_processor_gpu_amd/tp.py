@@ -161,3 +161,49 @@ class FrameBuffers:
         correlate2D(self.geom, self.p_clt, colors, scales[0], scales[1], scales[2], fat_zero,
                     self.num_corr, self.corr_idx, self.corr_stride, self.corr_radius, self.corrs,
                     stream)
+
+
+# texture parameters of the reference harness (src/dtt8x8.cu:791-814)
+TEXTURE_DEFAULTS = dict(colors=3, is_lwir=0, min_shot=10.0, scale_shot=3.0, diff_sigma=1.5,
+                        diff_threshold=10.0, min_agree=3.0, weight0=0.294118, weight1=0.117647,
+                        weight2=0.588235, dust_remove=1, keep_weights=0)
+
+
+def textures_accumulate(geom, woi, gpu_clt, num_texture_tiles, gpu_texture_indices,
+                        gpu_port_offsets, colors, is_lwir, min_shot, scale_shot, diff_sigma,
+                        diff_threshold, min_agree, weight0, weight1, weight2, dust_remove,
+                        keep_weights, texture_rbg_stride, gpu_texture_rbg, texture_stride,
+                        gpu_texture_tiles, stream=None):
+    """textures_accumulate (src/TileProcessor.cuh:2207-2585)."""
+    g = geom.c()
+    f = float
+    _lib.call("tp_textures_accumulate", ctypes.byref(g), _p(woi), _p(gpu_clt),
+              int(num_texture_tiles), _p(gpu_texture_indices), _p(gpu_port_offsets), int(colors),
+              int(is_lwir), f(min_shot), f(scale_shot), f(diff_sigma), f(diff_threshold),
+              f(min_agree), f(weight0), f(weight1), f(weight2), int(dust_remove), int(keep_weights),
+              int(texture_rbg_stride), _p(gpu_texture_rbg), int(texture_stride),
+              _p(gpu_texture_tiles), _stream(stream))
+
+
+def prepare_texture_list(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_tiles, woi,
+                         width, height, stream=None):
+    """prepare_texture_list (src/TileProcessor.cuh:1643-1713)."""
+    g = geom.c()
+    _lib.call("tp_prepare_texture_list", ctypes.byref(g), _p(gpu_tasks), int(num_tiles),
+              _p(gpu_texture_indices), _p(num_texture_tiles), _p(woi), int(width), int(height),
+              _stream(stream))
+
+
+def generate_RBGA(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_tiles, woi, width,
+                  height, gpu_clt, gpu_port_offsets, colors, is_lwir, min_shot, scale_shot,
+                  diff_sigma, diff_threshold, min_agree, weight0, weight1, weight2, dust_remove,
+                  keep_weights, texture_rbga_stride, gpu_texture_tiles, stream=None):
+    """generate_RBGA (src/TileProcessor.cuh:1446-1613), without dynamic parallelism."""
+    g = geom.c()
+    f = float
+    _lib.call("tp_generate_RBGA", ctypes.byref(g), _p(gpu_tasks), int(num_tiles),
+              _p(gpu_texture_indices), _p(num_texture_tiles), _p(woi), int(width), int(height),
+              _p(gpu_clt), _p(gpu_port_offsets), int(colors), int(is_lwir), f(min_shot),
+              f(scale_shot), f(diff_sigma), f(diff_threshold), f(min_agree), f(weight0),
+              f(weight1), f(weight2), int(dust_remove), int(keep_weights),
+              int(texture_rbga_stride), _p(gpu_texture_tiles), _stream(stream))
