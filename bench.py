@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=242, help="tile rows for the CPU baseline (0: skip)")
     ap.add_argument("--err-tiles", type=int, default=512)
+    ap.add_argument("--mode", choices=["frame", "copy"], default="frame",
+                    help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
+                         "FETCH_SIZE/WRITE_SIZE against known bytes)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -121,6 +124,17 @@ def main():
     import torch
     from tile_processor_gpu_amd import synth, tp
     from tile_processor_gpu_amd.geometry import corr_indices
+
+    if args.mode == "copy":
+        n = 1 << 28  # floats: 1 GiB
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(max(1, args.steps)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "copies": max(1, args.steps)}), flush=True)
+        return
 
     t0 = time.time()
     cfg = synth.config_full(seed=args.seed)

@@ -287,6 +287,38 @@ int tp_generate_RBGA(const tp_geometry *geom,
                      float *gpu_texture_tiles,
                      tp_stream_t stream);
 
+/* Six-pair combination (no counterpart in the reference: its Java host
+ * combines the per-pair tiles of correlate2D, src/TileProcessor.cuh:1421).
+ * Each pair's (2r+1)^2 tile is mapped onto the horizontal disparity axis
+ * (pairs 0,1 as is; 2,3 transposed; 4 (0,3): out(y,x) = in(x+y, x-y);
+ * 5 (2,1): out(y,x) = in(y-x, x+y), offsets from the centre) and the tiles
+ * of the pairs in pair_mask are summed into gpu_combined + tile *
+ * combined_stride.  Entries of one tile must be adjacent in the list (as in
+ * the tile-major list the reference harness builds); tiles without entries
+ * are not written.  A residual disparity dd peaks at (r, r + dd). */
+int tp_corr_combine(const tp_geometry *geom,
+                    size_t num_corr_tiles,
+                    const int *gpu_corr_indices,
+                    const float *gpu_corrs,
+                    size_t corr_stride,
+                    int corr_radius,
+                    int pair_mask,
+                    float *gpu_combined,
+                    size_t combined_stride,
+                    tp_stream_t stream);
+
+/* Sub-pixel disparity from the centre row of num_tiles combined tiles
+ * (no counterpart in the reference): integer argmax, then a 3-point
+ * parabola (mode 0) or the centroid of the positive samples within +-2
+ * (mode 1).  gpu_out: float[num_tiles][2] = (disparity in px, peak value). */
+int tp_corr_disparity(const float *gpu_tiles,
+                      size_t stride,
+                      int num_tiles,
+                      int corr_radius,
+                      int mode,
+                      float *gpu_out,
+                      tp_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

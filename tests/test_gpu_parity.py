@@ -223,3 +223,27 @@ def test_full_frame_tdac_and_correlation_peak(orc):
     fb.correlate(3, SCALES, FAT_ZERO)
     c = fb.corrs[:fb.num_corr].view(-1, 225)
     assert (c.argmax(1) == 112).all()
+
+
+def test_combine_and_disparity_match_oracle(small, orc):
+    g, idx = small["g"], small["idx"]
+    fb = small["fb"]
+    for cam in range(4):
+        fb.clt[cam].copy_(torch.from_numpy(small["clt64"][cam].astype(np.float32)))
+    fb.correlate(3, SCALES, FAT_ZERO)
+    n = 15
+    comb = torch.full((g.num_tiles, n * n + 1), -5.0, device="cuda:0")
+    tp.corr_combine(g, len(idx), fb.corr_idx, fb.corrs, fb.corr_stride, 7, 0x3F, comb, n * n + 1)
+    disp = torch.zeros((g.num_tiles, 2), device="cuda:0")
+    tp.corr_disparity(comb, n * n + 1, g.num_tiles, 7, 0, disp)
+    torch.cuda.synchronize()
+    ref_c = orc.correlate2D(g, small["clt64"], 3, SCALES, FAT_ZERO, idx, 7)
+    ref = orc.corr_combine(ref_c, 7)
+    got = comb.cpu().numpy()
+    assert rel_err(got[:, :n * n], ref) < TOL
+    assert (got[:, n * n] == -5.0).all()
+    d_ref = orc.corr_disparity(ref, 7, 0)
+    d = disp.cpu().numpy()
+    same_peak = np.abs(d[:, 1] - d_ref[:, 1]) <= 1e-5 * np.abs(d_ref[:, 1]).max()
+    assert same_peak.all()
+    assert np.abs(d[:, 0] - d_ref[:, 0]).max() < 1e-3

@@ -69,8 +69,17 @@ def test_texture_tiles_match_oracle(tex, orc, keep):
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     assert (got[:, nsl * 256:] == -9.0).all()
+    # port weights (slices 4..7) enter the output only through alpha =
+    # w^2 sum(weights) / 4; at tile corners they are ill-conditioned by
+    # construction (colour differences are scaled by 1 / w^4 before the
+    # exponential), so they are compared as w^2 * weight
+    w = np.sin(np.pi * (np.arange(16) + 0.5) / 16) ** 2
+    w2 = np.outer(w, w).ravel()
     for s in range(nsl):
-        fr, worst = frac_close(got[:, s * 256:(s + 1) * 256], ref[:, s * 256:(s + 1) * 256])
+        a, b = got[:, s * 256:(s + 1) * 256], ref[:, s * 256:(s + 1) * 256]
+        if 4 <= s < 8:
+            a, b = a * w2, b * w2
+        fr, worst = frac_close(a, b)
         print(f"slice {s}: close {fr:.6f}, worst {worst:.2e}")
         assert fr >= 0.9999, (s, fr, worst)
 
@@ -105,10 +114,11 @@ def test_generate_rbga_matches_oracle(tex, orc):
     lists = orc.sub_lists(g, ind, num)
     for a, b in zip(lists, lists_ref):
         assert sorted(a.tolist()) == sorted(b.tolist())
-    rows = (woi[3] + 1) * 8
+    rows, cols = (woi[3] + 1) * 8, (woi[2] + 1) * 8
     got = rbga[:4 * rows].reshape(4, rows, stride)
+    assert (got[:, :, cols:] == 7.0).all()  # outside the woi: untouched
     for s in range(4):
-        fr, worst = frac_close(got[s], rbga_ref[s])
+        fr, worst = frac_close(got[s][:, :cols], rbga_ref[s][:, :cols])
         print(f"rgba slice {s}: close {fr:.6f}, worst {worst:.2e}")
         assert fr >= 0.9999, (s, fr, worst)
 

@@ -8,7 +8,9 @@ import os
 
 from .geometry import CGeometry
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtp.so")
+# TP_LIB_PATH selects another build of the same library (tuning variants)
+LIB_PATH = os.environ.get("TP_LIB_PATH") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                          "libtp.so")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -28,6 +30,8 @@ PROTOTYPES = {
     "tp_textures_accumulate": [_G, _P, _P, _SZ, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F, _F, _F,
                                _I, _I, _SZ, _P, _SZ, _P, _P],
     "tp_prepare_texture_list": [_G, _P, _I, _P, _P, _P, _I, _I, _P],
+    "tp_corr_combine": [_G, _SZ, _P, _P, _SZ, _I, _I, _P, _SZ, _P],
+    "tp_corr_disparity": [_P, _SZ, _I, _I, _I, _P, _P],
     "tp_generate_RBGA": [_G, _P, _I, _P, _P, _P, _I, _I, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F,
                          _F, _F, _I, _I, _SZ, _P, _P],
 }

@@ -25,17 +25,26 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=True):
+def build(verbose=True, out=None, defines=()):
+    """Compile csrc/tp_api.hip (the whole library is one translation unit).
+    defines: extra -D tuning knobs (see csrc/tp_common.hpp); out: library
+    path (default: the in-tree libtp.so)."""
     src = os.path.join(CSRC, "tp_api.hip")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, src]
+    out = out or LIB
+    tmp = out + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-o", tmp, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    build()
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-o", "--out", default=None)
+    ap.add_argument("-D", "--define", action="append", default=[])
+    a = ap.parse_args()
+    build(out=a.out, defines=a.define)
     sys.exit(0)

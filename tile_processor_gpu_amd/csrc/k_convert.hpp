@@ -304,7 +304,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
     }
 }
 
-__global__ __launch_bounds__(192) void k_convert_correct_tiles(ConvertArgs a) {
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
+void k_convert_correct_tiles(ConvertArgs a) {
     __shared__ __align__(16) float lds[3][2][8][72];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;

@@ -42,7 +42,8 @@ __device__ __forceinline__ float lane_prev(float v) {
                                                       0xf, 0xf, false));
 }
 
-__global__ __launch_bounds__(256) void k_correlate2D(CorrArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
+void k_correlate2D(CorrArgs a) {
     __shared__ __align__(16) float lds[4][8][4][72];  // [wave][unit][quadrant][8x8+pad]
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;

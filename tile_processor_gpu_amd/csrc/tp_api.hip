@@ -13,6 +13,7 @@
 #include "k_correlate.hpp"
 #include "k_imclt.hpp"
 #include "k_texture.hpp"
+#include "k_combine.hpp"
 
 namespace {
 
@@ -89,7 +90,7 @@ int tp_convert_correct_tiles(const tp_geometry *geom, float **gpu_kernel_offsets
     a.dstride = dstride;
     a.num_tiles = num_tiles;
     a.lpf_mask = lpf_mask;
-    a.pairs_per_block = 4;
+    a.pairs_per_block = TP_CONVERT_PPB;
     const int npairs = (num_tiles + 1) / 2;
     const int grid = (npairs + a.pairs_per_block - 1) / a.pairs_per_block;
     hipLaunchKernelGGL(tp::k_convert_correct_tiles, dim3(grid), dim3(192), 0, as_stream(stream), a);
@@ -125,7 +126,7 @@ int tp_correlate2D(const tp_geometry *geom, float **gpu_clt, int colors, float s
     a.radius = corr_radius;
     a.corrs = gpu_corrs;
     a.ntiles = (geom->img_width / 8) * (geom->img_height / 8);
-    a.entries_per_block = 64;
+    a.entries_per_block = TP_CORR_EPB;
     const int grid = (int)((num_corr_tiles + a.entries_per_block - 1) / a.entries_per_block);
     hipLaunchKernelGGL(tp::k_correlate2D, dim3(grid), dim3(256), 0, as_stream(stream), a);
     return check_launch("correlate2D");
@@ -357,6 +358,56 @@ int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
         if ((rc = launch_textures(p, quarter, st))) return rc;
     }
     return TP_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int tp_corr_combine(const tp_geometry *geom, size_t num_corr_tiles, const int *gpu_corr_indices,
+                    const float *gpu_corrs, size_t corr_stride, int corr_radius, int pair_mask,
+                    float *gpu_combined, size_t combined_stride, tp_stream_t stream) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (!gpu_corr_indices || !gpu_corrs || !gpu_combined)
+        return fail(TP_EINVAL, "tp_corr_combine: NULL pointer argument%s", "");
+    if (corr_radius < 0 || corr_radius > TP_MAX_CORR_RADIUS)
+        return fail(TP_EINVAL, "tp_corr_combine: corr_radius must be 0..7%s", "");
+    const size_t n = 2 * corr_radius + 1;
+    if (corr_stride < n * n || combined_stride < n * n)
+        return fail(TP_EINVAL, "tp_corr_combine: stride < (2r+1)^2%s", "");
+    if (num_corr_tiles > (size_t)0x7fffffff) return fail(TP_EINVAL, "tp_corr_combine: too many entries%s", "");
+    g_err.clear();
+    if (!num_corr_tiles) return TP_OK;
+    tp::CombineArgs a;
+    a.num_corr = (int)num_corr_tiles;
+    a.idx = gpu_corr_indices;
+    a.corrs = gpu_corrs;
+    a.stride = corr_stride;
+    a.radius = corr_radius;
+    a.pair_mask = pair_mask;
+    a.ntiles = (geom->img_width / 8) * (geom->img_height / 8);
+    a.out = gpu_combined;
+    a.out_stride = combined_stride;
+    hipLaunchKernelGGL(tp::k_corr_combine, dim3((a.num_corr + 255) / 256), dim3(256), 0,
+                       as_stream(stream), a);
+    return check_launch("corr_combine");
+}
+
+int tp_corr_disparity(const float *gpu_tiles, size_t stride, int num_tiles, int corr_radius,
+                      int mode, float *gpu_out, tp_stream_t stream) {
+    if (!gpu_tiles || !gpu_out) return fail(TP_EINVAL, "tp_corr_disparity: NULL pointer argument%s", "");
+    if (corr_radius < 1 || corr_radius > TP_MAX_CORR_RADIUS)
+        return fail(TP_EINVAL, "tp_corr_disparity: corr_radius must be 1..7%s", "");
+    const size_t n = 2 * corr_radius + 1;
+    if (stride < n * n) return fail(TP_EINVAL, "tp_corr_disparity: stride < (2r+1)^2%s", "");
+    if (num_tiles < 0) return fail(TP_EINVAL, "tp_corr_disparity: num_tiles < 0%s", "");
+    if (mode != 0 && mode != 1) return fail(TP_EINVAL, "tp_corr_disparity: mode must be 0 or 1%s", "");
+    g_err.clear();
+    if (!num_tiles) return TP_OK;
+    hipLaunchKernelGGL(tp::k_corr_disparity, dim3((num_tiles + 255) / 256), dim3(256), 0,
+                       as_stream(stream), gpu_tiles, stride, num_tiles, corr_radius, mode, gpu_out);
+    return check_launch("corr_disparity");
 }
 
 }  // extern "C"

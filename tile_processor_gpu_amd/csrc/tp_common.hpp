@@ -11,6 +11,21 @@
 
 #include "../../include/tp.h"
 
+// Tuning knobs (build-time): minimum waves per SIMD the register allocator
+// must allow, and work per workgroup.
+#ifndef TP_CONVERT_WAVES
+#define TP_CONVERT_WAVES 4
+#endif
+#ifndef TP_CORR_WAVES
+#define TP_CORR_WAVES 2
+#endif
+#ifndef TP_CONVERT_PPB
+#define TP_CONVERT_PPB 4   // tile pairs per workgroup
+#endif
+#ifndef TP_CORR_EPB
+#define TP_CORR_EPB 64     // correlation entries per workgroup (multiple of 32)
+#endif
+
 namespace tp {
 
 // ---------------------------------------------------------------------

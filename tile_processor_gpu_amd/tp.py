@@ -207,3 +207,19 @@ def generate_RBGA(geom, gpu_tasks, num_tiles, gpu_texture_indices, num_texture_t
               f(scale_shot), f(diff_sigma), f(diff_threshold), f(min_agree), f(weight0),
               f(weight1), f(weight2), int(dust_remove), int(keep_weights),
               int(texture_rbga_stride), _p(gpu_texture_tiles), _stream(stream))
+
+
+def corr_combine(geom, num_corr_tiles, gpu_corr_indices, gpu_corrs, corr_stride, corr_radius,
+                 pair_mask, gpu_combined, combined_stride, stream=None):
+    """Sum the 6 pair tiles of each tile mapped onto the horizontal disparity
+    axis (no counterpart in the reference kernels)."""
+    g = geom.c()
+    _lib.call("tp_corr_combine", ctypes.byref(g), int(num_corr_tiles), _p(gpu_corr_indices),
+              _p(gpu_corrs), int(corr_stride), int(corr_radius), int(pair_mask),
+              _p(gpu_combined), int(combined_stride), _stream(stream))
+
+
+def corr_disparity(gpu_tiles, stride, num_tiles, corr_radius, mode, gpu_out, stream=None):
+    """(disparity, strength) per combined tile; mode 0 parabola, 1 centroid."""
+    _lib.call("tp_corr_disparity", _p(gpu_tiles), int(stride), int(num_tiles), int(corr_radius),
+              int(mode), _p(gpu_out), _stream(stream))
