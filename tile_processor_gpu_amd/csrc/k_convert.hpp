@@ -7,7 +7,9 @@
 // 16x16 -> 4 x 8x8 transform.  8 lanes own a unit (lane = one row or column
 // of the 8x8 folded block), so a wavefront runs 8 units side by side.  A
 // workgroup is 3 wavefronts, one per Bayer color, over 2 tiles x 4 cameras;
-// it loops over consecutive tile pairs of a contiguous chunk.
+// it loops over consecutive tile pairs of a contiguous chunk.  Tiles are
+// visited in 2x2 blocks (for a raster task list), so the 4 tiles that share
+// one 16x16-px aberration kernel cell meet in one workgroup.
 //
 // Per unit:
 //   1. lane = folded column k gathers its 8 (R, B) or 16 (G) Bayer samples,
@@ -25,21 +27,6 @@
 #include "tp_common.hpp"
 
 namespace tp {
-
-// per-lane window coefficients: w_s(n) = P(n) cos(pi s/16) + Q(n) sin(pi s/16)
-// for n = 0..15 (w_s(n) = sin(pi (n + 1/2 + s)/16)).
-__device__ __forceinline__ void win_pq(int n, float &P, float &Q) {
-    const int i = n < 8 ? n : 15 - n;
-    const int j = n < 8 ? 7 - n : n - 8;
-    float hi = 0.f, hj = 0.f;
-#pragma unroll
-    for (int t = 0; t < 8; t++) {  // runtime index into the 8-entry table
-        hi = (i == t) ? kTab.hw[t] : hi;
-        hj = (j == t) ? kTab.hw[t] : hj;
-    }
-    P = hi;
-    Q = n < 8 ? hj : -hj;
-}
 
 struct ConvertArgs {
     Geo g;
@@ -66,9 +53,68 @@ constexpr LpfTab make_lpf_tab() {
 // low-pass tables: lpf_data[R,B,G,mono], lpf_rb_corr, lpf_corr
 __constant__ LpfTab c_lpf = make_lpf_tab();
 
+// Processing order -> task index: 2x2 tile blocks within strips of two
+// tile rows (identity past the last complete strip).  A bijection on
+// [0, num_tiles) for any list; for a raster list of the whole grid the 4
+// tiles of one aberration kernel cell become consecutive.
+__device__ __forceinline__ int swizzle_2x2(int s, int TX, int num_tiles) {
+    const int strip = 2 * TX;
+    if (s >= (num_tiles / strip) * strip) return s;
+    const int st = s / strip, q = s - st * strip;
+    const int cp = q >> 2, sub = q & 3;
+    return st * strip + (sub >> 1) * TX + 2 * cp + (sub & 1);
+}
+
+// Per-lane constants of the horizontal fold: folded column li is fed by
+// source columns ca (the smaller) and cb; for each local column parity p:
+// the column and its window coefficients (w_s(c) = P cos(pi s/16) +
+// Q sin(pi s/16) = sin(pi (c + 1/2 + s)/16)) with the cosine- and
+// sine-branch fold signs applied.
+struct LaneFold {
+    int col0, col1;                // source column of local parity 0 / 1
+    float Pc0, Qc0, Pc1, Qc1;      // cosine branch (sign rcs)
+    float Ps0, Qs0, Ps1, Qs1;      // sine branch (sign rss)
+};
+
+__device__ __forceinline__ void win_coef(int c, float &P, float &Q) {
+    const int i = c < 8 ? c : 15 - c;
+    const int j = c < 8 ? 7 - c : c - 8;
+    float hi = 0.f, hj = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {  // runtime index into the 8-entry table (once per kernel)
+        hi = (i == t) ? kTab.hw[t] : hi;
+        hj = (j == t) ? kTab.hw[t] : hj;
+    }
+    P = hi;
+    Q = c < 8 ? hj : -hj;
+}
+
+__device__ __forceinline__ LaneFold lane_fold(int li) {
+    const int ca = li >= 4 ? li - 4 : 11 - li;
+    const int cb = li >= 4 ? 11 - li : 12 + li;  // opposite parity
+    float Pa, Qa, Pb, Qb;
+    win_coef(ca, Pa, Qa);
+    win_coef(cb, Pb, Qb);
+    const float sca = ca < 4 ? 1.f : -1.f, ssa = ca < 12 ? 1.f : -1.f;
+    const float scb = cb < 4 ? 1.f : -1.f, ssb = cb < 12 ? 1.f : -1.f;
+    const bool a_even = (ca & 1) == 0;
+    LaneFold f;
+    f.col0 = a_even ? ca : cb;
+    f.col1 = a_even ? cb : ca;
+    f.Pc0 = a_even ? sca * Pa : scb * Pb;
+    f.Qc0 = a_even ? sca * Qa : scb * Qb;
+    f.Pc1 = a_even ? scb * Pb : sca * Pa;
+    f.Qc1 = a_even ? scb * Qb : sca * Qa;
+    f.Ps0 = a_even ? ssa * Pa : ssb * Pb;
+    f.Qs0 = a_even ? ssa * Qa : ssb * Qb;
+    f.Ps1 = a_even ? ssb * Pb : ssa * Pa;
+    f.Qs1 = a_even ? ssb * Qb : ssa * Qa;
+    return f;
+}
+
 template <int COLOR>
-__device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c, float *lds_s,
-                                             int li, int cam, int task_num, bool valid) {
+__device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
+                                             float *lds_s, int li, int cam, int task_num, bool valid) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
     const tp_task *tk = a.tasks + task_num;
@@ -103,7 +149,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
     sincospif(sh * 0.0625f, &ahs, &ahc);
     sincospif(sv * 0.0625f, &avs, &avc);
 
-    // (3) windows.  Vertical: all 16 source rows (compile-time n).
+    // (3) windows.  Vertical: all 16 source rows (compile-time n); the fold
+    // sign (cosine branch) is folded in.
     float wv[16];
 #pragma unroll
     for (int n = 0; n < 16; n++) {
@@ -111,47 +158,40 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
         const float Q = n < 8 ? kTab.hw[7 - n] : -kTab.hw[n - 8];
         wv[n] = (float)fold_rcs(n) * fmaf(P, avc, Q * avs);
     }
-    // Horizontal: this lane's two source columns (folded column li).
-    const int ca = li >= 4 ? li - 4 : 11 - li;   // fold_src_a
-    const int cb = li >= 4 ? 11 - li : 12 + li;  // fold_src_b
-    float Pa, Qa, Pb, Qb;
-    win_pq(ca, Pa, Qa);
-    win_pq(cb, Pb, Qb);
-    const float wa = fmaf(Pa, ahc, Qa * ahs), wb = fmaf(Pb, ahc, Qb * ahs);
-    // column of each local parity, with signed cosine / sine windows
-    const int pa = ca & 1;
-    int colp[2];
-    float whc[2], whs[2];
-    colp[pa] = ca;
-    colp[pa ^ 1] = cb;
-    whc[pa] = (ca < 4 ? 1.f : -1.f) * wa;
-    whc[pa ^ 1] = (cb < 4 ? 1.f : -1.f) * wb;
-    whs[pa] = (ca < 12 ? 1.f : -1.f) * wa;
-    whs[pa ^ 1] = (cb < 12 ? 1.f : -1.f) * wb;
+    // Horizontal: this lane's two source columns, by local parity.
+    const float whc0 = fmaf(lf.Pc0, ahc, lf.Qc0 * ahs), whc1 = fmaf(lf.Pc1, ahc, lf.Qc1 * ahs);
+    const float whs0 = fmaf(lf.Ps0, ahc, lf.Qs0 * ahs), whs1 = fmaf(lf.Ps1, ahc, lf.Qs1 * ahs);
 
+    // Pixel offsets (32-bit).  Interior windows (the usual case) are linear
+    // in the row; windows reaching past the right/bottom edge read the
+    // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
     const float *img = a.images[cam];
-    const size_t ds = a.dstride;
-    auto pix = [&](int r, int c) -> float {
-        int y = itly + r, x = itlx + c;
-        y = min(y, g.H - 2 + (y & 1));
-        x = min(x, g.W - 2 + (x & 1));
-        return img[(size_t)y * ds + x];
+    const int ds = (int)a.dstride;
+    const bool inside = (itlx + 15 < g.W) && (itly + 15 < g.H);
+    auto rowoff = [&](int r) -> int {
+        const int y = itly + r;
+        return (inside ? y : min(y, g.H - 2 + (y & 1))) * ds;
     };
+    auto coloff = [&](int c) -> int {
+        const int x = itlx + c;
+        return inside ? x : min(x, g.W - 2 + (x & 1));
+    };
+    const int x0 = coloff(lf.col0), x1 = coloff(lf.col1);
 
     // (4) fold: f[l] = folded (l, li)
     float f[8], fs[8];
     if (!color1) {
         const int pc = (itlx & 1) ^ (1 ^ color0);
         const int pr = (itly & 1) ^ color0;
-        const int c = colp[pc];
-        const float wc = whc[pc];
+        const int xo = pc ? x1 : x0;
+        const float wc = pc ? whc1 : whc0;
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
             const bool use_a = (ra & 1) == pr;
             const int r = use_a ? ra : rb;
             const float w = use_a ? wv[ra] : wv[rb];
-            f[l] = pix(r, c) * w * wc;
+            f[l] = img[rowoff(r) + xo] * w * wc;
         }
     } else {
         // G sites: (row + col + itlx + itly) even; both source rows used
@@ -159,17 +199,17 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
-            const int pca = (ra + S) & 1, pcb = (rb + S) & 1;
-            const float da = pix(ra, colp[pca]) * wv[ra];
-            const float db = pix(rb, colp[pcb]) * wv[rb];
-            f[l] = fmaf(da, whc[pca], db * whc[pcb]);
-            fs[l] = fmaf(da, whs[pca], db * whs[pcb]);
+            const int pca = (ra + S) & 1;  // column parity for row ra; row rb takes the other
+            const float da = img[rowoff(ra) + (pca ? x1 : x0)] * wv[ra];
+            const float db = img[rowoff(rb) + (pca ? x0 : x1)] * wv[rb];
+            f[l] = fmaf(da, pca ? whc1 : whc0, db * (pca ? whc0 : whc1));
+            fs[l] = fmaf(da, pca ? whs1 : whs0, db * (pca ? whs0 : whs1));
         }
     }
 
     // (5) vertical DCT-IV (column li)
-    dtt8<0>(f);
-    if (color1) dtt8<0>(fs);
+    dct4_8(f);
+    if (color1) dct4_8(fs);
 
     // (6) transpose through LDS: lane li writes column li as row li of the
     // staging block, then reads its row li (element k from lane k).
@@ -190,8 +230,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
     wave_lds_sync();
 
     // (7) horizontal DCT-IV (and DST-IV for G)
-    dtt8<0>(cc);
-    if (color1) dtt8<1>(sc);
+    dct4_8(cc);
+    if (color1) dst4_8(sc);
 
     // (8) replicate quadrants, :3457-3497.  Lane li is row l = li.
     const float nd = (float)(1 - ((((itlx & 1) ^ (itly & 1) ^ (1 ^ color1))) << 1));
@@ -224,24 +264,24 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, float *lds_c,
         }
     }
 
-    // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827
+    // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827;
+    // one half row (4 bins) of the 4 kernel quadrants at a time
     {
         const float4 *kp = (const float4 *)(kern + li * 8);
-        float kq[4][8];
 #pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
-            const float4 u = kp[qq * 16], v = kp[qq * 16 + 1];
-            kq[qq][0] = u.x; kq[qq][1] = u.y; kq[qq][2] = u.z; kq[qq][3] = u.w;
-            kq[qq][4] = v.x; kq[qq][5] = v.y; kq[qq][6] = v.z; kq[qq][7] = v.w;
-        }
+        for (int h = 0; h < 2; h++) {
+            const float4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float c0 = q0[k], c1 = q1[k], c2 = q2[k], c3 = q3[k];
-            const float k0 = kq[0][k], k1 = kq[1][k], k2 = kq[2][k], k3 = kq[3][k];
-            q0[k] = fmaf(c3, k3, fmaf(-c2, k2, fmaf(-c1, k1, c0 * k0)));
-            q1[k] = fmaf(-c2, k3, fmaf(-c3, k2, fmaf(c0, k1, c1 * k0)));
-            q2[k] = fmaf(-c1, k3, fmaf(c0, k2, fmaf(-c3, k1, c2 * k0)));
-            q3[k] = fmaf(c0, k3, fmaf(c1, k2, fmaf(c2, k1, c3 * k0)));
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = 4 * h + kk;
+                auto el = [kk](const float4 &v) { return kk == 0 ? v.x : kk == 1 ? v.y : kk == 2 ? v.z : v.w; };
+                const float k0 = el(K0), k1 = el(K1), k2 = el(K2), k3 = el(K3);
+                const float c0 = q0[k], c1 = q1[k], c2 = q2[k], c3 = q3[k];
+                q0[k] = fmaf(c3, k3, fmaf(-c2, k2, fmaf(-c1, k1, c0 * k0)));
+                q1[k] = fmaf(-c2, k3, fmaf(-c3, k2, fmaf(c0, k1, c1 * k0)));
+                q2[k] = fmaf(-c1, k3, fmaf(c0, k2, fmaf(-c3, k1, c2 * k0)));
+                q3[k] = fmaf(c0, k3, fmaf(c1, k2, fmaf(c2, k1, c3 * k0)));
+            }
         }
     }
 
@@ -318,17 +358,18 @@ void k_convert_correct_tiles(ConvertArgs a) {
     const int p1 = min(npairs, p0 + a.pairs_per_block);
     float *lds_c = &lds[wave][0][unit][0];
     float *lds_s = &lds[wave][1][unit][0];
+    const LaneFold lf = lane_fold(li);
     for (int p = p0; p < p1; p++) {
-        int task_num = p * 2 + (unit >> 2);
-        bool valid = task_num < a.num_tiles;
-        if (!valid) task_num = a.num_tiles - 1;
+        const int s = p * 2 + (unit >> 2);
+        bool valid = s < a.num_tiles;
+        const int task_num = valid ? swizzle_2x2(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
         valid = valid && (a.tasks[task_num].task != 0);
         if (wave == 0)
-            convert_unit<0>(a, lds_c, lds_s, li, cam, task_num, valid);
+            convert_unit<0>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
         else if (wave == 1)
-            convert_unit<1>(a, lds_c, lds_s, li, cam, task_num, valid);
+            convert_unit<1>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
         else
-            convert_unit<2>(a, lds_c, lds_s, li, cam, task_num, valid);
+            convert_unit<2>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
     }
 }
 

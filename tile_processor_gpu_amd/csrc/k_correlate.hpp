@@ -15,7 +15,8 @@
 //   3. horizontal DCT-II / DST-II in registers, transpose through LDS,
 //      vertical DCT-II / DST-II (lane = column j);
 //   4. unfold to (2r+1)^2: column j combines its own column with column j-1
-//      (one DPP row shift) and writes output columns r+j and r-j.
+//      (one DPP row shift) into output columns r+j and r-j, staged in LDS;
+//      the wavefront then stores its 8 output rows with contiguous stores.
 #pragma once
 
 #include "tp_common.hpp"
@@ -42,6 +43,7 @@ __device__ __forceinline__ float lane_prev(float v) {
                                                       0xf, 0xf, false));
 }
 
+template <int COLORS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
 void k_correlate2D(CorrArgs a) {
     __shared__ __align__(16) float lds[4][8][4][72];  // [wave][unit][quadrant][8x8+pad]
@@ -54,6 +56,13 @@ void k_correlate2D(CorrArgs a) {
     const int e1 = min(a.num_corr, e0 + a.entries_per_block);
     const int r = a.radius, n = 2 * r + 1;
     const float fz2 = a.fat_zero * a.fat_zero;
+    // output o = lane + 64 t of a (2r+1)^2 tile <- staging offset (-1: none)
+    int src_off[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int o = lane + 64 * t;
+        src_off[t] = o < n * n ? ((o / n) + 7 - r) * 15 + (o % n) + 7 - r : -1;
+    }
 
     for (int eb = e0 + wave * 8; eb < e1; eb += 32) {
         int e = eb + unit;
@@ -74,34 +83,37 @@ void k_correlate2D(CorrArgs a) {
 #pragma unroll
         for (int k = 0; k < 8; k++) c0[k] = c1[k] = c2[k] = c3[k] = 0.f;
 
-        for (int color = 0; color < a.colors; color++) {
+#pragma unroll
+        for (int color = 0; color < COLORS; color++) {
             const size_t off = ((size_t)tile * TP_NUM_COLORS + color) * 256 + li * 8;
             const float4 *pa = (const float4 *)(a.clt[cam1] + off);
             const float4 *pb = (const float4 *)(a.clt[cam2] + off);
-            float A[4][8], B[4][8];
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) {
-                float4 u = pa[qq * 16], w = pa[qq * 16 + 1];
-                A[qq][0] = u.x; A[qq][1] = u.y; A[qq][2] = u.z; A[qq][3] = u.w;
-                A[qq][4] = w.x; A[qq][5] = w.y; A[qq][6] = w.z; A[qq][7] = w.w;
-                u = pb[qq * 16]; w = pb[qq * 16 + 1];
-                B[qq][0] = u.x; B[qq][1] = u.y; B[qq][2] = u.z; B[qq][3] = u.w;
-                B[qq][4] = w.x; B[qq][5] = w.y; B[qq][6] = w.z; B[qq][7] = w.w;
-            }
             const float sc = a.scale[color];
+            // two halves of the row (k = 0..3, 4..7) keep 32 loaded floats live
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const float a0 = A[0][k], a1 = A[1][k], a2 = A[2][k], a3 = A[3][k];
-                const float b0 = B[0][k], b1 = B[1][k], b2 = B[2][k], b3 = B[3][k];
-                // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
-                const float r0 = fmaf(a3, b3, fmaf(a2, b2, fmaf(a1, b1, a0 * b0)));
-                const float r1 = fmaf(a2, b3, fmaf(-a3, b2, fmaf(a0, b1, -a1 * b0)));
-                const float r2 = fmaf(a1, b3, fmaf(a0, b2, fmaf(-a3, b1, -a2 * b0)));
-                const float r3 = fmaf(a0, b3, fmaf(-a1, b2, fmaf(-a2, b1, a3 * b0)));
-                c0[k] = fmaf(sc, r0, c0[k]);
-                c1[k] = fmaf(sc, r1, c1[k]);
-                c2[k] = fmaf(sc, r2, c2[k]);
-                c3[k] = fmaf(sc, r3, c3[k]);
+            for (int h = 0; h < 2; h++) {
+                float4 A[4], B[4];
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) {
+                    A[qq] = pa[qq * 16 + h];
+                    B[qq] = pb[qq * 16 + h];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int k = 4 * h + kk;
+                    auto el = [kk](const float4 &v) { return kk == 0 ? v.x : kk == 1 ? v.y : kk == 2 ? v.z : v.w; };
+                    const float a0 = el(A[0]), a1 = el(A[1]), a2 = el(A[2]), a3 = el(A[3]);
+                    const float b0 = el(B[0]), b1 = el(B[1]), b2 = el(B[2]), b3 = el(B[3]);
+                    // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
+                    const float r0 = fmaf(a3, b3, fmaf(a2, b2, fmaf(a1, b1, a0 * b0)));
+                    const float r1 = fmaf(a2, b3, fmaf(-a3, b2, fmaf(a0, b1, -a1 * b0)));
+                    const float r2 = fmaf(a1, b3, fmaf(a0, b2, fmaf(-a3, b1, -a2 * b0)));
+                    const float r3 = fmaf(a0, b3, fmaf(-a1, b2, fmaf(-a2, b1, a3 * b0)));
+                    c0[k] = fmaf(sc, r0, c0[k]);
+                    c1[k] = fmaf(sc, r1, c1[k]);
+                    c2[k] = fmaf(sc, r2, c2[k]);
+                    c3[k] = fmaf(sc, r3, c3[k]);
+                }
             }
             if (color == 1) {  // R + B low-pass, :1263-1304
 #pragma unroll
@@ -119,10 +131,10 @@ void k_correlate2D(CorrArgs a) {
             c0[k] *= s; c1[k] *= s; c2[k] *= s; c3[k] *= s;
         }
         // horizontal DCT-II (q0, q2) / DST-II (q1, q3) on row li
-        dtt8<2>(c0);
-        dtt8<3>(c1);
-        dtt8<2>(c2);
-        dtt8<3>(c3);
+        dct2_8(c0);
+        dst2_8(c1);
+        dct2_8(c2);
+        dst2_8(c3);
         // transpose: row li -> LDS, read column li
         float *L = &lds[wave][unit][0][0];
         {
@@ -143,10 +155,10 @@ void k_correlate2D(CorrArgs a) {
         }
         wave_lds_sync();
         // vertical DCT-II (q0, q1) / DST-II (q2, q3) on column li
-        dtt8<2>(c0);
-        dtt8<2>(c1);
-        dtt8<3>(c2);
-        dtt8<3>(c3);
+        dct2_8(c0);
+        dct2_8(c1);
+        dst2_8(c2);
+        dst2_8(c3);
         // unfold, :2950-3008.  Lane li is column j; q1, q3 come from column j-1.
         float p1[8], p3[8];
 #pragma unroll
@@ -154,24 +166,36 @@ void k_correlate2D(CorrArgs a) {
             p1[i] = lane_prev(c1[i]);
             p3[i] = lane_prev(c3[i]);
         }
-        const int j = li;
-        if (valid && j <= r) {
-            float *out = a.corrs + (size_t)e * a.stride;
+        // stage the full 15x15 unfold of each unit in its LDS block (fixed
+        // layout, centre (7, 7)); the wavefront then stores the (2r+1)^2
+        // window of its 8 units with contiguous stores
+        {
+            const int j = li;
             const float s4 = 0.25f;
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                if (i > r) break;
                 const float a0 = c0[i];
                 const float a1 = j ? p1[i] : 0.f;
                 const float a2 = i ? c2[i - 1] : 0.f;
                 const float a3 = (i && j) ? p3[i - 1] : 0.f;
                 const float t1 = a0 + a2, t2 = a0 - a2, u1 = a1 + a3, u2 = a1 - a3;
-                out[(r + i) * n + r + j] = s4 * (t1 + u1);
-                out[(r + i) * n + r - j] = s4 * (t1 - u1);
-                out[(r - i) * n + r + j] = s4 * (t2 + u2);
-                out[(r - i) * n + r - j] = s4 * (t2 - u2);
+                L[(7 + i) * 15 + 7 + j] = s4 * (t1 + u1);
+                L[(7 + i) * 15 + 7 - j] = s4 * (t1 - u1);
+                L[(7 - i) * 15 + 7 + j] = s4 * (t2 + u2);
+                L[(7 - i) * 15 + 7 - j] = s4 * (t2 - u2);
             }
         }
+        wave_lds_sync();
+        const int vflag = valid ? 1 : 0;
+        for (int u = 0; u < 8; u++) {
+            if (!__shfl(vflag, u * 8)) continue;
+            const float *src = &lds[wave][u][0][0];
+            float *dst = a.corrs + (size_t)(eb + u) * a.stride;
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (src_off[t] >= 0) dst[lane + 64 * t] = src[src_off[t]];
+        }
+        wave_lds_sync();
     }
 }
 

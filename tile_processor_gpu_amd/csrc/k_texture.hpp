@@ -156,13 +156,17 @@ __global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
     if (a.keep_weights) {
         float sw = 0.f;
         for (int col = 0; col < colors; col++) {
-            float s1 = 0.f, s2 = 0.f;
+            // Spread of the 4 cameras around their mean, two-pass: the same
+            // value as (s0 s2 - s1^2) / s0^2 (src/TileProcessor.cuh:4286-4296)
+            // without that form's cancellation when the cameras nearly agree.
+            const float mean = 0.25f * ((D[0][col] + D[1][col]) + (D[2][col] + D[3][col]));
+            float ss = 0.f;
 #pragma unroll
             for (int cam = 0; cam < 4; cam++) {
-                s1 += D[cam][col];
-                s2 = fmaf(D[cam][col], D[cam][col], s2);
+                const float e = D[cam][col] - mean;
+                ss = fmaf(e, e, ss);
             }
-            const float mse = fmaxf(0.f, (4.f * s2 - s1 * s1) * 0.0625f);
+            const float mse = 0.25f * ss;
             crms[col] = sqrtf(mse);
             sw = fmaf(a.w[col], mse, sw);
         }

@@ -94,6 +94,10 @@ struct HostTables {
     float dtt[4][8][8];
     float hw[8];
     float lpf[6][64];
+    float tw_pre[4][2];   // fast DCT-IV: e^{-i pi (4m+1)/32}
+    float tw_post[4][2];  // fast DCT-IV: e^{-i pi q/8} / 2
+    float c2odd[4][4];    // fast DCT-II odd part: cos(pi (2n+1)(2m+1)/16) / 2
+    float c2even[3];      // cos(pi 2/16)/2, cos(pi 4/16)/2, cos(pi 6/16)/2
 };
 
 constexpr HostTables make_tables() {
@@ -102,6 +106,16 @@ constexpr HostTables make_tables() {
         for (int k = 0; k < 8; k++)
             for (int n = 0; n < 8; n++) t.dtt[kd][k][n] = (float)dtt_coef(kd, k, n);
     for (int i = 0; i < 8; i++) t.hw[i] = (float)hwin(i);
+    for (int m = 0; m < 4; m++) {
+        t.tw_pre[m][0] = (float)cx_cos(-kPi * (4 * m + 1) / 32.0);
+        t.tw_pre[m][1] = (float)cx_sin(-kPi * (4 * m + 1) / 32.0);
+        t.tw_post[m][0] = (float)(0.5 * cx_cos(-kPi * m / 8.0));
+        t.tw_post[m][1] = (float)(0.5 * cx_sin(-kPi * m / 8.0));
+        for (int n = 0; n < 4; n++) t.c2odd[m][n] = (float)(0.5 * cx_cos(kPi * (2 * n + 1) * (2 * m + 1) / 16.0));
+    }
+    t.c2even[0] = (float)(0.5 * cx_cos(kPi * 2 / 16.0));
+    t.c2even[1] = (float)(0.5 * cx_cos(kPi * 4 / 16.0));
+    t.c2even[2] = (float)(0.5 * cx_cos(kPi * 6 / 16.0));
     const double sig[6] = {0.9, 0.9, 0.6, 0.4, 0.5, 0.9};
     for (int w = 0; w < 6; w++) {
         double f[8] = {};
@@ -178,6 +192,91 @@ __device__ __forceinline__ void dtt8(float (&x)[8]) {
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = y[k];
+}
+
+// 8-point DCT-IV, y[k] = 1/2 sum_n x[n] cos(pi/8 (n+1/2)(k+1/2)), through a
+// 4-point complex DFT: v[m] = x[2m] + i x[7-2m] twiddled by
+// e^{-i pi (4m+1)/32}, DFT (only +-1, +-i factors), twiddled by
+// e^{-i pi q/8}; then y[2q] = Re/2, y[7-2q] = -Im/2.  46 operations
+// instead of 64.
+__device__ __forceinline__ void dct4_8(float (&x)[8]) {
+    float tr[4], ti[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const float vr = x[2 * m], vi = x[7 - 2 * m];
+        const float wr = kTab.tw_pre[m][0], wi = kTab.tw_pre[m][1];
+        tr[m] = fmaf(vr, wr, -vi * wi);
+        ti[m] = fmaf(vr, wi, vi * wr);
+    }
+    // 4-point DFT T[q] = sum_m t[m] e^{-2 pi i m q / 4}
+    const float s0r = tr[0] + tr[2], s0i = ti[0] + ti[2];
+    const float d0r = tr[0] - tr[2], d0i = ti[0] - ti[2];
+    const float s1r = tr[1] + tr[3], s1i = ti[1] + ti[3];
+    const float d1r = tr[1] - tr[3], d1i = ti[1] - ti[3];
+    float Tr[4], Ti[4];
+    Tr[0] = s0r + s1r; Ti[0] = s0i + s1i;
+    Tr[2] = s0r - s1r; Ti[2] = s0i - s1i;
+    Tr[1] = d0r + d1i; Ti[1] = d0i - d1r;   // d0 - i d1
+    Tr[3] = d0r - d1i; Ti[3] = d0i + d1r;   // d0 + i d1
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float pr = kTab.tw_post[q][0], pi = kTab.tw_post[q][1];
+        float ur, ui;
+        if (q == 0) {
+            ur = 0.5f * Tr[0];
+            ui = 0.5f * Ti[0];
+        } else {
+            ur = fmaf(Tr[q], pr, -Ti[q] * pi);
+            ui = fmaf(Tr[q], pi, Ti[q] * pr);
+        }
+        x[2 * q] = ur;
+        x[7 - 2 * q] = -ui;
+    }
+}
+
+// 8-point DST-IV, y[k] = 1/2 sum_n x[n] sin(pi/8 (n+1/2)(k+1/2)) =
+// (-1)^k DCT-IV of the reversed input.
+__device__ __forceinline__ void dst4_8(float (&x)[8]) {
+    float t[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) t[n] = x[7 - n];
+    dct4_8(t);
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = (k & 1) ? -t[k] : t[k];
+}
+
+// 8-point DCT-II, y[k] = 1/2 sum_n x[n] cos(pi (2n+1) k / 16) (the DC term
+// not divided by sqrt 2), by the even/odd split: the even outputs are a
+// 4-point DCT-II of x[n] + x[7-n], the odd ones a 4x4 product with
+// x[n] - x[7-n].  36 operations instead of 64.
+__device__ __forceinline__ void dct2_8(float (&x)[8]) {
+    const float u0 = x[0] + x[7], u1 = x[1] + x[6], u2 = x[2] + x[5], u3 = x[3] + x[4];
+    const float v0 = x[0] - x[7], v1 = x[1] - x[6], v2 = x[2] - x[5], v3 = x[3] - x[4];
+    const float a0 = u0 + u3, a1 = u1 + u2, b0 = u0 - u3, b1 = u1 - u2;
+    const float h2 = kTab.c2even[0], h4 = kTab.c2even[1], h6 = kTab.c2even[2];
+    x[0] = 0.5f * (a0 + a1);
+    x[4] = h4 * (a0 - a1);
+    x[2] = fmaf(h2, b0, h6 * b1);
+    x[6] = fmaf(h6, b0, -h2 * b1);
+    const float v[4] = {v0, v1, v2, v3};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        float s = kTab.c2odd[m][0] * v[0];
+#pragma unroll
+        for (int n = 1; n < 4; n++) s = fmaf(kTab.c2odd[m][n], v[n], s);
+        x[2 * m + 1] = s;
+    }
+}
+
+// 8-point DST-II, y[k] = 1/2 sum_n x[n] sin(pi (2n+1)(k+1) / 16): the DCT-II
+// of (-1)^n x[n], read backwards.
+__device__ __forceinline__ void dst2_8(float (&x)[8]) {
+    float t[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) t[n] = (n & 1) ? -x[n] : x[n];
+    dct2_8(t);
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = t[7 - k];
 }
 
 // complex multiply helper
