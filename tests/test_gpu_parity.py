@@ -55,6 +55,26 @@ def test_convert_matches_oracle(small):
             assert rel_err(a, c) < TOL, (cam, color, rel_err(a, c))
 
 
+def test_convert_other_grid(orc):
+    """A 34 x 26 tile grid (tile rows not a multiple of 4, tile columns not a
+    multiple of 4): the forward CLT's visiting order must still cover every
+    tile exactly once."""
+    g = Geometry(272, 208)
+    cfg = synth.config_small(seed=5, geom=g)
+    print(f"[gpu parity] odd grid {g}", flush=True)
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=7)
+    fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"],
+              corr_indices(g, cfg["tasks"]))
+    for c in fb.clt:
+        c.fill_(np.nan)
+    fb.convert()
+    got = gpu_clt(fb)
+    ref = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], cfg["tasks"])
+    for cam in range(4):
+        assert np.isfinite(got[cam]).all(), cam
+        assert rel_err(got[cam], ref[cam]) < TOL, (cam, rel_err(got[cam], ref[cam]))
+
+
 def test_convert_is_deterministic(small):
     fb = small["fb"]
     fb.convert()

@@ -8,8 +8,9 @@
 // of the 8x8 folded block), so a wavefront runs 8 units side by side.  A
 // workgroup is 3 wavefronts, one per Bayer color, over 2 tiles x 4 cameras;
 // it loops over consecutive tile pairs of a contiguous chunk.  Tiles are
-// visited in 2x2 blocks (for a raster task list), so the 4 tiles that share
-// one 16x16-px aberration kernel cell meet in one workgroup.
+// visited by column pairs within strips of TP_STRIP_ROWS rows (for a raster
+// task list), so the tiles that share one 16x16-px aberration kernel cell
+// meet in one workgroup or in workgroups that run at the same time.
 //
 // Per unit:
 //   1. lane = folded column k gathers its 8 (R, B) or 16 (G) Bayer samples,
@@ -53,16 +54,28 @@ constexpr LpfTab make_lpf_tab() {
 // low-pass tables: lpf_data[R,B,G,mono], lpf_rb_corr, lpf_corr
 __constant__ LpfTab c_lpf = make_lpf_tab();
 
-// Processing order -> task index: 2x2 tile blocks within strips of two
-// tile rows (identity past the last complete strip).  A bijection on
-// [0, num_tiles) for any list; for a raster list of the whole grid the 4
-// tiles of one aberration kernel cell become consecutive.
-__device__ __forceinline__ int swizzle_2x2(int s, int TX, int num_tiles) {
-    const int strip = 2 * TX;
+// Processing order -> task index.  The grid is cut into strips of
+// TP_STRIP_ROWS tile rows; within a strip, tiles are visited by column pairs
+// (rows of a pair top to bottom), so that tiles sharing a 16x16-px
+// aberration kernel cell meet in one workgroup or in workgroups that run at
+// the same time.  Identity past the last complete strip; a last odd column
+// is visited top to bottom.  A bijection on [0, num_tiles) for any TX.
+__device__ __forceinline__ int swizzle_strip(int s, int TX, int num_tiles) {
+    constexpr int SH = TP_STRIP_ROWS;
+    const int strip = SH * TX;
     if (s >= (num_tiles / strip) * strip) return s;
     const int st = s / strip, q = s - st * strip;
-    const int cp = q >> 2, sub = q & 3;
-    return st * strip + (sub >> 1) * TX + 2 * cp + (sub & 1);
+    const int full = (TX >> 1) * (2 * SH);
+    int r, c;
+    if (q < full) {
+        const int cp = q / (2 * SH), sub = q - cp * (2 * SH);
+        r = sub >> 1;
+        c = 2 * cp + (sub & 1);
+    } else {
+        r = q - full;
+        c = TX - 1;
+    }
+    return st * strip + r * TX + c;
 }
 
 // Per-lane constants of the horizontal fold: folded column li is fed by
@@ -129,10 +142,10 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     const float kdx = (cx - (float)(ktx << 4)) + 8.0f;
     const float kdy = (cy - (float)(kty << 4)) + 8.0f;
     const size_t kfi = (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + COLOR;
-    const float *ex = a.koffs[cam] + kfi * 8;
+    gfloat *ex = gptr(a.koffs[cam] + kfi * 8);
     const float offx = fmaf(ex[5], kdy, fmaf(ex[4], kdx, ex[0]));
     const float offy = fmaf(ex[7], kdy, fmaf(ex[6], kdx, ex[1]));
-    const float *kern = a.kernels[cam] + kfi * 256;
+    const float *kern = a.kernels[cam] + kfi * 256;  // read through gptr4
 
     // (2) integer top-left, residual shift, :3182-3247
     const float px = (cx - 8.0f) - offx;
@@ -165,7 +178,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // Pixel offsets (32-bit).  Interior windows (the usual case) are linear
     // in the row; windows reaching past the right/bottom edge read the
     // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
-    const float *img = a.images[cam];
+    gfloat *img = gptr(a.images[cam]);
     const int ds = (int)a.dstride;
     const bool inside = (itlx + 15 < g.W) && (itly + 15 < g.H);
     auto rowoff = [&](int r) -> int {
@@ -267,14 +280,14 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827;
     // one half row (4 bins) of the 4 kernel quadrants at a time
     {
-        const float4 *kp = (const float4 *)(kern + li * 8);
+        gf32x4 *kp = gptr4(kern + li * 8);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const float4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
+            const f32x4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
                 const int k = 4 * h + kk;
-                auto el = [kk](const float4 &v) { return kk == 0 ? v.x : kk == 1 ? v.y : kk == 2 ? v.z : v.w; };
+                auto el = [kk](const f32x4 &v) { return v[kk]; };
                 const float k0 = el(K0), k1 = el(K1), k2 = el(K2), k3 = el(K3);
                 const float c0 = q0[k], c1 = q1[k], c2 = q2[k], c3 = q3[k];
                 q0[k] = fmaf(c3, k3, fmaf(-c2, k2, fmaf(-c1, k1, c0 * k0)));
@@ -332,15 +345,14 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     const int tx = txy & 0xfff, ty = txy >> 16;
     if (valid && tx < g.TX && ty >= 0 && ty < g.TY) {  // out-of-grid tiles are skipped
         float *dst = a.clt[cam] + ((size_t)(ty * g.TX + tx) * TP_NUM_COLORS + COLOR) * 256 + li * 8;
-        float4 *d4 = (float4 *)dst;
-        d4[0] = make_float4(q0[0], q0[1], q0[2], q0[3]);
-        d4[1] = make_float4(q0[4], q0[5], q0[6], q0[7]);
-        d4[16] = make_float4(q1[0], q1[1], q1[2], q1[3]);
-        d4[17] = make_float4(q1[4], q1[5], q1[6], q1[7]);
-        d4[32] = make_float4(q2[0], q2[1], q2[2], q2[3]);
-        d4[33] = make_float4(q2[4], q2[5], q2[6], q2[7]);
-        d4[48] = make_float4(q3[0], q3[1], q3[2], q3[3]);
-        d4[49] = make_float4(q3[4], q3[5], q3[6], q3[7]);
+        store4(dst, q0[0], q0[1], q0[2], q0[3]);
+        store4(dst + 4, q0[4], q0[5], q0[6], q0[7]);
+        store4(dst + 64, q1[0], q1[1], q1[2], q1[3]);
+        store4(dst + 68, q1[4], q1[5], q1[6], q1[7]);
+        store4(dst + 128, q2[0], q2[1], q2[2], q2[3]);
+        store4(dst + 132, q2[4], q2[5], q2[6], q2[7]);
+        store4(dst + 192, q3[0], q3[1], q3[2], q3[3]);
+        store4(dst + 196, q3[4], q3[5], q3[6], q3[7]);
     }
 }
 
@@ -362,7 +374,7 @@ void k_convert_correct_tiles(ConvertArgs a) {
     for (int p = p0; p < p1; p++) {
         const int s = p * 2 + (unit >> 2);
         bool valid = s < a.num_tiles;
-        const int task_num = valid ? swizzle_2x2(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
+        const int task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
         valid = valid && (a.tasks[task_num].task != 0);
         if (wave == 0)
             convert_unit<0>(a, lf, lds_c, lds_s, li, cam, task_num, valid);

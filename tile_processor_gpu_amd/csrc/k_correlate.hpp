@@ -86,13 +86,13 @@ void k_correlate2D(CorrArgs a) {
 #pragma unroll
         for (int color = 0; color < COLORS; color++) {
             const size_t off = ((size_t)tile * TP_NUM_COLORS + color) * 256 + li * 8;
-            const float4 *pa = (const float4 *)(a.clt[cam1] + off);
-            const float4 *pb = (const float4 *)(a.clt[cam2] + off);
+            gf32x4 *pa = gptr4(a.clt[cam1] + off);
+            gf32x4 *pb = gptr4(a.clt[cam2] + off);
             const float sc = a.scale[color];
             // two halves of the row (k = 0..3, 4..7) keep 32 loaded floats live
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                float4 A[4], B[4];
+                f32x4 A[4], B[4];
 #pragma unroll
                 for (int qq = 0; qq < 4; qq++) {
                     A[qq] = pa[qq * 16 + h];
@@ -101,7 +101,7 @@ void k_correlate2D(CorrArgs a) {
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
                     const int k = 4 * h + kk;
-                    auto el = [kk](const float4 &v) { return kk == 0 ? v.x : kk == 1 ? v.y : kk == 2 ? v.z : v.w; };
+                    auto el = [kk](const f32x4 &v) { return v[kk]; };
                     const float a0 = el(A[0]), a1 = el(A[1]), a2 = el(A[2]), a3 = el(A[3]);
                     const float b0 = el(B[0]), b1 = el(B[1]), b2 = el(B[2]), b3 = el(B[3]);
                     // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
@@ -193,7 +193,7 @@ void k_correlate2D(CorrArgs a) {
             float *dst = a.corrs + (size_t)(eb + u) * a.stride;
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                if (src_off[t] >= 0) dst[lane + 64 * t] = src[src_off[t]];
+                if (src_off[t] >= 0) store1(dst + lane + 64 * t, src[src_off[t]]);
         }
         wave_lds_sync();
     }

@@ -25,6 +25,12 @@
 #ifndef TP_CORR_EPB
 #define TP_CORR_EPB 64     // correlation entries per workgroup (multiple of 32)
 #endif
+#ifndef TP_STRIP_ROWS
+#define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
+#endif
+#ifndef TP_NT_STORE
+#define TP_NT_STORE 1      // write-once outputs bypass L2 retention (non-temporal)
+#endif
 
 namespace tp {
 
@@ -164,6 +170,38 @@ inline Geo make_geo(const tp_geometry &g) {
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
     const int full = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
     return x * full + (x < rem ? x : rem) + i;
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Global-memory views of generic pointers.  Pointers read from the
+// per-camera pointer arrays are generic to the compiler, which then emits
+// flat loads; those count against the LDS wait counter as well and
+// serialise LDS waits behind outstanding HBM loads.
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const f32x4 gf32x4;
+__device__ __forceinline__ gfloat *gptr(const float *p) { return (gfloat *)p; }
+__device__ __forceinline__ gf32x4 *gptr4(const float *p) { return (gf32x4 *)p; }
+
+// 16-byte store of an output that is not read again by this kernel: with
+// TP_NT_STORE it is marked non-temporal, so it does not displace the inputs
+// that neighbouring workgroups re-read from L2.
+__device__ __forceinline__ void store4(float *p, float a, float b, float c, float d) {
+    typedef __attribute__((address_space(1))) f32x4 gv4;
+    const f32x4 v = {a, b, c, d};
+#if TP_NT_STORE
+    __builtin_nontemporal_store(v, (gv4 *)p);
+#else
+    *(gv4 *)p = v;
+#endif
+}
+__device__ __forceinline__ void store1(float *p, float v) {
+    typedef __attribute__((address_space(1))) float gv1;
+#if TP_NT_STORE
+    __builtin_nontemporal_store(v, (gv1 *)p);
+#else
+    *(gv1 *)p = v;
+#endif
 }
 
 // Order LDS traffic between lanes of one wavefront.
