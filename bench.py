@@ -29,6 +29,25 @@ HBM_PEAK_GBS = 8000.0
 FAT_ZERO = 0.05
 SCALES = (0.25, 0.25, 0.5)
 RADIUS = 7
+# per-kernel HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+# this benchmark, calibrated on a 1 GiB device copy (written by
+# tile_processor_gpu_amd.rocprof_summary; the matching .md gives the commands)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clt_corr_full_frame_traffic.json")
+KERNEL_SYMBOLS = {"convert_correct_tiles": "tp::k_convert_correct_tiles",
+                  "correlate2D": "void tp::k_correlate2D<3>"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of one kernel from the committed counter summary,
+    or None when there is none."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)[KERNEL_SYMBOLS[kernel]]
+    except (OSError, KeyError, ValueError):
+        return None
+    return {"bytes": int(t["fetch_bytes"] + t["write_bytes"]), "read": int(t["fetch_bytes"]),
+            "written": int(t["write_bytes"]), "profiled_avg_us": round(t["avg_us"], 2),
+            "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
 def log(*a):
@@ -185,6 +204,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBps"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
                 "traffic": None}
+    tr = measured_traffic(dom)
+    if tr is not None:
+        roofline["traffic"] = tr["bytes"]
+        roofline["traffic_detail"] = tr
     frame_bytes = conv_b + corr_b
     stage = {k: {"ms": round(v["ms"], 4), "GBps": round(v["GBps"], 1)} for k, v in kernels.items()}
 

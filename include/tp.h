@@ -319,6 +319,112 @@ int tp_corr_disparity(const float *gpu_tiles,
                       float *gpu_out,
                       tp_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Fused and stream forms (no counterpart in the reference).  They compute
+ * what tp_convert_correct_tiles followed by tp_correlate2D computes, with the
+ * same per-tile arithmetic, but keep each tile's CLT in on-chip memory
+ * between the two steps.  Correlation rows are dense per task: the pair p of
+ * task t goes to row 6 t + p (for a full task list with all pairs set, the
+ * same order as the tile-major correlation list the reference harness
+ * builds, src/dtt8x8.cu:453-472); rows of pairs not set in a task are not
+ * written.
+ * ---------------------------------------------------------------------- */
+
+/* Launch configuration (NULL: defaults). */
+typedef struct tp_fused_config {
+    int32_t tiles_per_item;   /* tiles per work item, a multiple of tiles_per_group (default 8) */
+    int32_t tiles_per_group;  /* tiles held in LDS at once: 2 or 4 (default 2) */
+    int32_t blocks_per_cu;    /* stream form: resident workgroups per CU, 1..8 (default 4) */
+    int32_t dct;              /* forward DCT-IV/DST-IV: 0 FP32 butterflies (default) */
+} tp_fused_config;
+
+/* One frame of the stream form; an array of these lives in device memory.
+ * images / kernels / kernel_offsets / clt are device arrays of TP_NUM_CAMS
+ * device pointers as for tp_convert_correct_tiles; clt may be NULL (CLT not
+ * stored).  item_start is written by the library. */
+typedef struct tp_frame {
+    const tp_task *tasks;
+    float **images;
+    float **kernels;
+    float **kernel_offsets;
+    float **clt;
+    float *corrs;
+    int32_t num_tiles;
+    int32_t item_start;
+} tp_frame;
+
+/* Forward CLT + correlation of one frame.  Arguments as for
+ * tp_convert_correct_tiles and tp_correlate2D; gpu_clt may be NULL. */
+int tp_convert_correlate(const tp_geometry *geom,
+                         float **gpu_kernel_offsets,
+                         float **gpu_kernels,
+                         float **gpu_images,
+                         const tp_task *gpu_tasks,
+                         float **gpu_clt,
+                         size_t dstride,
+                         int num_tiles,
+                         int lpf_mask,
+                         int colors,
+                         float scale0,
+                         float scale1,
+                         float scale2,
+                         float fat_zero,
+                         size_t corr_stride,
+                         int corr_radius,
+                         float *gpu_corrs,
+                         const tp_fused_config *config,
+                         tp_stream_t stream);
+
+/* Forward CLT + correlation of a stream of frames (each with its own task
+ * list, images, kernels and outputs) in one persistent launch: a fixed grid
+ * of blocks_per_cu workgroups per CU takes work items from a global counter.
+ * gpu_work: device int32[2] (queue counter, item count), reset by the call.
+ * gpu_stamps: NULL or device uint64[nframes][2], set to each frame's first
+ * item start and last item end (GPU wall clock, wall_clock_hz ticks/s). */
+int tp_stream_convert_correlate(const tp_geometry *geom,
+                                tp_frame *gpu_frames,
+                                int nframes,
+                                int32_t *gpu_work,
+                                uint64_t *gpu_stamps,
+                                size_t dstride,
+                                int lpf_mask,
+                                int colors,
+                                float scale0,
+                                float scale1,
+                                float scale2,
+                                float fat_zero,
+                                size_t corr_stride,
+                                int corr_radius,
+                                const tp_fused_config *config,
+                                tp_stream_t stream);
+
+/* The same stream launch captured once into an executable HIP graph
+ * (set-up kernel + persistent kernel, no memset nodes); replaying it
+ * re-reads every buffer, so new frame data can be uploaded between
+ * launches. */
+typedef void *tp_graph_t;
+int tp_stream_graph_create(const tp_geometry *geom,
+                           tp_frame *gpu_frames,
+                           int nframes,
+                           int32_t *gpu_work,
+                           uint64_t *gpu_stamps,
+                           size_t dstride,
+                           int lpf_mask,
+                           int colors,
+                           float scale0,
+                           float scale1,
+                           float scale2,
+                           float fat_zero,
+                           size_t corr_stride,
+                           int corr_radius,
+                           const tp_fused_config *config,
+                           tp_graph_t *graph);
+int tp_graph_launch(tp_graph_t graph, tp_stream_t stream);
+int tp_graph_destroy(tp_graph_t graph);
+
+/* Ticks per second of the clock behind gpu_stamps. */
+double tp_wall_clock_hz(void);
+
 #ifdef __cplusplus
 }
 #endif

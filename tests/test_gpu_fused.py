@@ -1,0 +1,186 @@
+"""GPU parity of the fused (forward CLT + correlation in one pass) and stream
+(persistent launch over many frames, optionally replayed from a captured
+graph) forms against the staged kernels and the FP64 oracle.
+
+Tolerance: max |x - x64| / max |x64| <= 1e-5 against the oracle, as for the
+staged path; against the staged GPU kernels (the same per-tile arithmetic,
+differently scheduled) <= 1e-6.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tile_processor_gpu_amd import synth, tp
+from tile_processor_gpu_amd.geometry import corr_indices
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+TOL_STAGED = 1e-6
+FZ = 0.05
+SCALES = (0.25, 0.25, 0.5)
+SENTINEL = -7.0
+
+
+def _frame(cfg, radius=7):
+    g = cfg["geom"]
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=radius, max_corr=6 * len(cfg["tasks"]))
+    fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], corr_indices(g, cfg["tasks"]))
+    return fb
+
+
+def _staged(fb, colors=3):
+    """Staged results: CLT and the correlation list output."""
+    fb.convert()
+    fb.correlate(colors, SCALES, FZ)
+    torch.cuda.synchronize()
+    return [c.clone() for c in fb.clt], fb.corrs[:fb.num_corr].clone()
+
+
+def _dense_rows(cfg):
+    """Row (6 task + pair) of every correlation-list entry."""
+    t = cfg["tasks"]
+    cm = (t["task"] >> 4) & 63
+    bits = (cm[:, None] >> np.arange(6)[None, :]) & 1
+    ti, pi = np.nonzero(bits)
+    return 6 * ti + pi
+
+
+@pytest.fixture(scope="module")
+def small():
+    cfg = synth.config_small(seed=21)
+    print(f"[gpu fused] small config {cfg['geom']}", flush=True)
+    fb = _frame(cfg)
+    clt, corr = _staged(fb)
+    return dict(cfg=cfg, fb=fb, clt=clt, corr=corr)
+
+
+def test_fused_matches_staged_and_oracle(small, orc):
+    fb, cfg = small["fb"], small["cfg"]
+    for c in fb.clt:
+        c.fill_(np.nan)
+    fb.corrs.fill_(SENTINEL)
+    fb.fused(3, SCALES, FZ, store_clt=True)
+    torch.cuda.synchronize()
+    for cam in range(4):
+        a, b = fb.clt[cam].cpu().numpy(), small["clt"][cam].cpu().numpy()
+        assert rel_err(a, b) <= TOL_STAGED, (cam, rel_err(a, b))
+    got = fb.corrs[:6 * fb.num_tiles].cpu().numpy()
+    ref = small["corr"].cpu().numpy()
+    assert rel_err(got, ref) <= TOL_STAGED, rel_err(got, ref)
+    g = cfg["geom"]
+    clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], cfg["tasks"])
+    o = orc.correlate2D(g, clt64, 3, SCALES, FZ, corr_indices(g, cfg["tasks"]), 7)
+    assert rel_err(got, o) < TOL, rel_err(got, o)
+
+
+@pytest.mark.parametrize("gt,tpi", [(2, 2), (2, 8), (4, 4), (4, 16), (2, 64)])
+def test_fused_configs(small, gt, tpi):
+    fb = small["fb"]
+    fb.corrs.fill_(SENTINEL)
+    fb.fused(3, SCALES, FZ, config=dict(tiles_per_item=tpi, tiles_per_group=gt))
+    torch.cuda.synchronize()
+    got = fb.corrs[:6 * fb.num_tiles].cpu().numpy()
+    assert rel_err(got, small["corr"].cpu().numpy()) <= TOL_STAGED, (gt, tpi)
+
+
+def test_fused_mono_and_radius(orc):
+    cfg = synth.config_small(seed=22)
+    g = cfg["geom"]
+    fb = _frame(cfg, radius=4)
+    _, ref = _staged(fb, colors=1)
+    fb.corrs.fill_(SENTINEL)
+    fb.fused(1, SCALES, FZ, config=dict(tiles_per_item=4, tiles_per_group=4))
+    torch.cuda.synchronize()
+    got = fb.corrs[:6 * fb.num_tiles].cpu().numpy()
+    assert rel_err(got[:, :81], ref.cpu().numpy()[:, :81]) <= TOL_STAGED
+    assert (got[:, 81:] == SENTINEL).all()  # past (2r+1)^2: untouched
+
+
+def test_fused_sparse_bits_and_nop(orc):
+    """Tasks with some pairs off, NOP tasks and out-of-grid tasks: only the
+    rows of set pairs are written, and they match the staged kernels."""
+    cfg = synth.config_small(seed=23)
+    rng = np.random.default_rng(3)
+    t = cfg["tasks"].copy()
+    n = len(t)
+    bits = rng.integers(0, 64, n)
+    t["task"] = (t["task"] & 0xF) | (bits << 4)
+    t["task"][rng.choice(n, 40, replace=False)] = 0
+    bad = rng.choice(n, 5, replace=False)
+    t["txy"][bad] = (1000 << 16) | 3  # ty outside the grid
+    cfg = dict(cfg, tasks=t)
+    fb = _frame(cfg)
+    _, ref = _staged(fb)
+    fb.corrs.fill_(SENTINEL)
+    fb.fused(3, SCALES, FZ)
+    torch.cuda.synchronize()
+    got = fb.corrs[:6 * n].cpu().numpy()
+    idx = corr_indices(cfg["geom"], t)
+    rows = _dense_rows(cfg)
+    ok = np.ones(len(idx), bool)
+    ok[np.isin(rows // 6, bad)] = False  # staged list entries of out-of-grid tiles
+    r = ref.cpu().numpy()
+    assert rel_err(got[rows[ok]], r[ok]) <= TOL_STAGED
+    written = np.zeros(6 * n, bool)
+    written[rows[ok]] = True
+    assert (got[~written] == SENTINEL).all()
+
+
+@pytest.fixture(scope="module")
+def stream():
+    frames = synth.config_stream(nframes=12, seed=31)
+    print(f"[gpu fused] stream of {len(frames)} frames, tiles "
+          f"{[len(f['tasks']) for f in frames]}", flush=True)
+    fbs = [_frame(f) for f in frames]
+    refs = []
+    for fb in fbs:
+        fb.fused(3, SCALES, FZ)
+        torch.cuda.synchronize()
+        refs.append(fb.corrs[:6 * fb.num_tiles].clone())
+    return dict(frames=frames, fbs=fbs, refs=refs)
+
+
+@pytest.mark.parametrize("cfg", [dict(tiles_per_item=8, tiles_per_group=2, blocks_per_cu=4),
+                                 dict(tiles_per_item=2, tiles_per_group=2, blocks_per_cu=1),
+                                 dict(tiles_per_item=16, tiles_per_group=4, blocks_per_cu=2)])
+def test_stream_matches_single_frames(stream, cfg):
+    fbs = stream["fbs"]
+    for fb in fbs:
+        fb.corrs.fill_(SENTINEL)
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=cfg)
+    run.launch()
+    torch.cuda.synchronize()
+    for i, fb in enumerate(fbs):
+        got = fb.corrs[:6 * fb.num_tiles].cpu().numpy()
+        assert not (got == SENTINEL).any(), (cfg, i)
+        assert rel_err(got, stream["refs"][i].cpu().numpy()) <= TOL_STAGED, (cfg, i)
+    times = run.frame_times_s()
+    assert (times[:, 1] >= times[:, 0]).all() and times[:, 0].min() == 0.0
+    assert int(run.work[0].item()) >= int(run.work[1].item())  # queue drained
+
+
+def test_stream_graph_replays(stream):
+    """A captured graph replayed several times, with new image data uploaded
+    between replays and eager work in between: every replay matches the
+    eager stream launch on the same data."""
+    fbs = stream["fbs"][:5]
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=4))
+    run.capture()
+    rng = np.random.default_rng(9)
+    for rep in range(3):
+        k = rep % len(fbs)
+        fbs[k].images[1].mul_(float(rng.uniform(0.5, 2.0)))
+        for fb in fbs:
+            fb.corrs.fill_(SENTINEL)
+        run.replay()
+        torch.cuda.synchronize()
+        got = [fb.corrs[:6 * fb.num_tiles].clone() for fb in fbs]
+        for fb in fbs:
+            fb.corrs.fill_(SENTINEL)
+        run.launch()
+        torch.cuda.synchronize()
+        for i, fb in enumerate(fbs):
+            assert torch.equal(got[i], fb.corrs[:6 * fb.num_tiles]), (rep, i)
+    run.release()

@@ -125,13 +125,16 @@ __device__ __forceinline__ LaneFold lane_fold(int li) {
     return f;
 }
 
+// One unit (tile task_num, camera cam, color COLOR) up to the optional LPF:
+// on return lane li holds row l = li of the 4 quadrants in q0..q3.
 template <int COLOR>
 __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
-                                             float *lds_s, int li, int cam, int task_num, bool valid) {
+                                             float *lds_s, int li, int cam, int task_num,
+                                             float (&q0)[8], float (&q1)[8], float (&q2)[8],
+                                             float (&q3)[8]) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
     const tp_task *tk = a.tasks + task_num;
-    const int txy = tk->txy;
     const float cx = tk->xy[cam][0], cy = tk->xy[cam][1];
 
     // (1) kernel cell, :3129-3134
@@ -248,7 +251,6 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
 
     // (8) replicate quadrants, :3457-3497.  Lane li is row l = li.
     const float nd = (float)(1 - ((((itlx & 1) ^ (itly & 1) ^ (1 ^ color1))) << 1));
-    float q0[8], q1[8], q2[8], q3[8];
     if (!color1) {
         const float nh = (float)(1 - (((itlx & 1) ^ (1 ^ COLOR)) << 1));
         float m[8];
@@ -341,19 +343,39 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
             q0[k] *= lv; q1[k] *= lv; q2[k] *= lv; q3[k] *= lv;
         }
     }
-    // (12) store [q][l][k], :3600-3630
+}
+
+// Tile index of a task, or -1 for a task outside the tile grid (skipped).
+__device__ __forceinline__ int task_tile(const Geo &g, int txy) {
     const int tx = txy & 0xfff, ty = txy >> 16;
-    if (valid && tx < g.TX && ty >= 0 && ty < g.TY) {  // out-of-grid tiles are skipped
-        float *dst = a.clt[cam] + ((size_t)(ty * g.TX + tx) * TP_NUM_COLORS + COLOR) * 256 + li * 8;
-        store4(dst, q0[0], q0[1], q0[2], q0[3]);
-        store4(dst + 4, q0[4], q0[5], q0[6], q0[7]);
-        store4(dst + 64, q1[0], q1[1], q1[2], q1[3]);
-        store4(dst + 68, q1[4], q1[5], q1[6], q1[7]);
-        store4(dst + 128, q2[0], q2[1], q2[2], q2[3]);
-        store4(dst + 132, q2[4], q2[5], q2[6], q2[7]);
-        store4(dst + 192, q3[0], q3[1], q3[2], q3[3]);
-        store4(dst + 196, q3[4], q3[5], q3[6], q3[7]);
-    }
+    return (tx < g.TX && ty >= 0 && ty < g.TY) ? ty * g.TX + tx : -1;
+}
+
+// Row li of the 4 quadrants of one 8x8x4 CLT block ([q][l][k]) at dst.
+__device__ __forceinline__ void store_clt_rows(float *dst, int li, const float (&q0)[8],
+                                               const float (&q1)[8], const float (&q2)[8],
+                                               const float (&q3)[8]) {
+    dst += li * 8;
+    store4(dst, q0[0], q0[1], q0[2], q0[3]);
+    store4(dst + 4, q0[4], q0[5], q0[6], q0[7]);
+    store4(dst + 64, q1[0], q1[1], q1[2], q1[3]);
+    store4(dst + 68, q1[4], q1[5], q1[6], q1[7]);
+    store4(dst + 128, q2[0], q2[1], q2[2], q2[3]);
+    store4(dst + 132, q2[4], q2[5], q2[6], q2[7]);
+    store4(dst + 192, q3[0], q3[1], q3[2], q3[3]);
+    store4(dst + 196, q3[4], q3[5], q3[6], q3[7]);
+}
+
+template <int COLOR>
+__device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const LaneFold &lf,
+                                                   float *lds_c, float *lds_s, int li, int cam,
+                                                   int task_num, bool valid) {
+    float q0[8], q1[8], q2[8], q3[8];
+    convert_unit<COLOR>(a, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+    // (12) store [q][l][k], :3600-3630; out-of-grid tiles are skipped
+    const int tile = task_tile(a.g, a.tasks[task_num].txy);
+    if (valid && tile >= 0)
+        store_clt_rows(a.clt[cam] + ((size_t)tile * TP_NUM_COLORS + COLOR) * 256, li, q0, q1, q2, q3);
 }
 
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
@@ -377,11 +399,11 @@ void k_convert_correct_tiles(ConvertArgs a) {
         const int task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
         valid = valid && (a.tasks[task_num].task != 0);
         if (wave == 0)
-            convert_unit<0>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<0>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
         else if (wave == 1)
-            convert_unit<1>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<1>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
         else
-            convert_unit<2>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<2>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
     }
 }
 

@@ -43,6 +43,135 @@ __device__ __forceinline__ float lane_prev(float v) {
                                                       0xf, 0xf, false));
 }
 
+// Accumulate sum_color scale * conj(A) B for row li of one entry
+// (correlateAccumulateTiles :2829-2890, R+B low-pass :1263-1304).
+// load(color, which, qq, h) returns floats 4h..4h+3 of row li of quadrant qq
+// of camera `which` (0: first camera of the pair, 1: second).
+template <int COLORS, class Load>
+__device__ __forceinline__ void corr_accumulate(Load load, const float *scale, int li, float (&c0)[8],
+                                                float (&c1)[8], float (&c2)[8], float (&c3)[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) c0[k] = c1[k] = c2[k] = c3[k] = 0.f;
+#pragma unroll
+    for (int color = 0; color < COLORS; color++) {
+        const float sc = scale[color];
+        // two halves of the row (k = 0..3, 4..7) keep 32 loaded floats live
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            f32x4 A[4], B[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) {
+                A[qq] = load(color, 0, qq, h);
+                B[qq] = load(color, 1, qq, h);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = 4 * h + kk;
+                const float a0 = A[0][kk], a1 = A[1][kk], a2 = A[2][kk], a3 = A[3][kk];
+                const float b0 = B[0][kk], b1 = B[1][kk], b2 = B[2][kk], b3 = B[3][kk];
+                // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
+                const float r0 = fmaf(a3, b3, fmaf(a2, b2, fmaf(a1, b1, a0 * b0)));
+                const float r1 = fmaf(a2, b3, fmaf(-a3, b2, fmaf(a0, b1, -a1 * b0)));
+                const float r2 = fmaf(a1, b3, fmaf(a0, b2, fmaf(-a3, b1, -a2 * b0)));
+                const float r3 = fmaf(a0, b3, fmaf(-a1, b2, fmaf(-a2, b1, a3 * b0)));
+                c0[k] = fmaf(sc, r0, c0[k]);
+                c1[k] = fmaf(sc, r1, c1[k]);
+                c2[k] = fmaf(sc, r2, c2[k]);
+                c3[k] = fmaf(sc, r3, c3[k]);
+            }
+        }
+        if (color == 1) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float lv = c_lpf.v[4][li * 8 + k];
+                c0[k] *= lv; c1[k] *= lv; c2[k] *= lv; c3[k] *= lv;
+            }
+        }
+    }
+}
+
+// Normalise, back-transform and unfold one entry (row li of its 8 lanes):
+// amplitude normalisation with fat zero and lpf_corr (:2915-2943,
+// :1337-1347), horizontal DCT-II/DST-II, transpose through L, vertical
+// DCT-II/DST-II, unfold (:2950-3008) into the full 15x15 tile staged in L
+// (row-major, centre (7, 7)).  L: 4 * 72 floats private to the entry.
+__device__ __forceinline__ void corr_finish(float (&c0)[8], float (&c1)[8], float (&c2)[8],
+                                            float (&c3)[8], float *L, int li, float fz2) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float s2 = fmaf(c3[k], c3[k], fmaf(c2[k], c2[k], fmaf(c1[k], c1[k], fmaf(c0[k], c0[k], fz2))));
+        const float s = __frsqrt_rn(s2) * c_lpf.v[5][li * 8 + k];
+        c0[k] *= s; c1[k] *= s; c2[k] *= s; c3[k] *= s;
+    }
+    // horizontal DCT-II (q0, q2) / DST-II (q1, q3) on row li
+    dct2_8(c0);
+    dst2_8(c1);
+    dct2_8(c2);
+    dst2_8(c3);
+    // transpose: row li -> LDS, read column li
+    {
+        float *q[4] = {c0, c1, c2, c3};
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
+            *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        c0[i] = L[0 * 72 + i * 8 + li];
+        c1[i] = L[1 * 72 + i * 8 + li];
+        c2[i] = L[2 * 72 + i * 8 + li];
+        c3[i] = L[3 * 72 + i * 8 + li];
+    }
+    wave_lds_sync();
+    // vertical DCT-II (q0, q1) / DST-II (q2, q3) on column li
+    dct2_8(c0);
+    dct2_8(c1);
+    dst2_8(c2);
+    dst2_8(c3);
+    // unfold.  Lane li is column j; q1, q3 come from column j-1.
+    float p1[8], p3[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        p1[i] = lane_prev(c1[i]);
+        p3[i] = lane_prev(c3[i]);
+    }
+    const int j = li;
+    const float s4 = 0.25f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const float a0 = c0[i];
+        const float a1 = j ? p1[i] : 0.f;
+        const float a2 = i ? c2[i - 1] : 0.f;
+        const float a3 = (i && j) ? p3[i - 1] : 0.f;
+        const float t1 = a0 + a2, t2 = a0 - a2, u1 = a1 + a3, u2 = a1 - a3;
+        L[(7 + i) * 15 + 7 + j] = s4 * (t1 + u1);
+        L[(7 + i) * 15 + 7 - j] = s4 * (t1 - u1);
+        L[(7 - i) * 15 + 7 + j] = s4 * (t2 + u2);
+        L[(7 - i) * 15 + 7 - j] = s4 * (t2 - u2);
+    }
+}
+
+// Offsets into the staged 15x15 tile of output elements lane + 64 t of a
+// (2r+1)^2 tile (-1: past the end).
+__device__ __forceinline__ void corr_out_offsets(int lane, int r, int *src_off) {
+    const int n = 2 * r + 1;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int o = lane + 64 * t;
+        src_off[t] = o < n * n ? ((o / n) + 7 - r) * 15 + (o % n) + 7 - r : -1;
+    }
+}
+
+__device__ __forceinline__ int pair_cam1(int pair) {
+    return pair == 0 ? 0 : pair == 1 ? 2 : pair == 2 ? 0 : pair == 3 ? 1 : pair == 4 ? 0 : 2;
+}
+__device__ __forceinline__ int pair_cam2(int pair) {
+    return pair == 0 ? 1 : pair == 1 ? 3 : pair == 2 ? 2 : pair == 3 ? 3 : pair == 4 ? 3 : 1;
+}
+
 template <int COLORS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
 void k_correlate2D(CorrArgs a) {
@@ -54,15 +183,9 @@ void k_correlate2D(CorrArgs a) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int e0 = blk * a.entries_per_block;
     const int e1 = min(a.num_corr, e0 + a.entries_per_block);
-    const int r = a.radius, n = 2 * r + 1;
     const float fz2 = a.fat_zero * a.fat_zero;
-    // output o = lane + 64 t of a (2r+1)^2 tile <- staging offset (-1: none)
     int src_off[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int o = lane + 64 * t;
-        src_off[t] = o < n * n ? ((o / n) + 7 - r) * 15 + (o % n) + 7 - r : -1;
-    }
+    corr_out_offsets(lane, a.radius, src_off);
 
     for (int eb = e0 + wave * 8; eb < e1; eb += 32) {
         int e = eb + unit;
@@ -76,115 +199,15 @@ void k_correlate2D(CorrArgs a) {
             pair = 0;
             tile = 0;
         }
-        const int cam1 = pair == 0 ? 0 : pair == 1 ? 2 : pair == 2 ? 0 : pair == 3 ? 1 : pair == 4 ? 0 : 2;
-        const int cam2 = pair == 0 ? 1 : pair == 1 ? 3 : pair == 2 ? 2 : pair == 3 ? 3 : pair == 4 ? 3 : 1;
-
+        const size_t off = (size_t)tile * TP_NUM_COLORS * 256 + li * 8;
+        const float *ca = a.clt[pair_cam1(pair)] + off, *cb = a.clt[pair_cam2(pair)] + off;
+        auto load = [&](int color, int which, int qq, int h) -> f32x4 {
+            return gptr4((which ? cb : ca) + color * 256)[qq * 16 + h];
+        };
         float c0[8], c1[8], c2[8], c3[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) c0[k] = c1[k] = c2[k] = c3[k] = 0.f;
-
-#pragma unroll
-        for (int color = 0; color < COLORS; color++) {
-            const size_t off = ((size_t)tile * TP_NUM_COLORS + color) * 256 + li * 8;
-            gf32x4 *pa = gptr4(a.clt[cam1] + off);
-            gf32x4 *pb = gptr4(a.clt[cam2] + off);
-            const float sc = a.scale[color];
-            // two halves of the row (k = 0..3, 4..7) keep 32 loaded floats live
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                f32x4 A[4], B[4];
-#pragma unroll
-                for (int qq = 0; qq < 4; qq++) {
-                    A[qq] = pa[qq * 16 + h];
-                    B[qq] = pb[qq * 16 + h];
-                }
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const int k = 4 * h + kk;
-                    auto el = [kk](const f32x4 &v) { return v[kk]; };
-                    const float a0 = el(A[0]), a1 = el(A[1]), a2 = el(A[2]), a3 = el(A[3]);
-                    const float b0 = el(B[0]), b1 = el(B[1]), b2 = el(B[2]), b3 = el(B[3]);
-                    // conj(A) * B, bicomplex with i^2 = j^2 = -1, ij = ji
-                    const float r0 = fmaf(a3, b3, fmaf(a2, b2, fmaf(a1, b1, a0 * b0)));
-                    const float r1 = fmaf(a2, b3, fmaf(-a3, b2, fmaf(a0, b1, -a1 * b0)));
-                    const float r2 = fmaf(a1, b3, fmaf(a0, b2, fmaf(-a3, b1, -a2 * b0)));
-                    const float r3 = fmaf(a0, b3, fmaf(-a1, b2, fmaf(-a2, b1, a3 * b0)));
-                    c0[k] = fmaf(sc, r0, c0[k]);
-                    c1[k] = fmaf(sc, r1, c1[k]);
-                    c2[k] = fmaf(sc, r2, c2[k]);
-                    c3[k] = fmaf(sc, r3, c3[k]);
-                }
-            }
-            if (color == 1) {  // R + B low-pass, :1263-1304
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const float lv = c_lpf.v[4][li * 8 + k];
-                    c0[k] *= lv; c1[k] *= lv; c2[k] *= lv; c3[k] *= lv;
-                }
-            }
-        }
-        // normalise with fat zero, then lpf_corr, :2915-2943, :1337-1347
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float s2 = fmaf(c3[k], c3[k], fmaf(c2[k], c2[k], fmaf(c1[k], c1[k], fmaf(c0[k], c0[k], fz2))));
-            const float s = __frsqrt_rn(s2) * c_lpf.v[5][li * 8 + k];
-            c0[k] *= s; c1[k] *= s; c2[k] *= s; c3[k] *= s;
-        }
-        // horizontal DCT-II (q0, q2) / DST-II (q1, q3) on row li
-        dct2_8(c0);
-        dst2_8(c1);
-        dct2_8(c2);
-        dst2_8(c3);
-        // transpose: row li -> LDS, read column li
+        corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
         float *L = &lds[wave][unit][0][0];
-        {
-            float *q[4] = {c0, c1, c2, c3};
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) {
-                *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
-                *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
-            }
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            c0[i] = L[0 * 72 + i * 8 + li];
-            c1[i] = L[1 * 72 + i * 8 + li];
-            c2[i] = L[2 * 72 + i * 8 + li];
-            c3[i] = L[3 * 72 + i * 8 + li];
-        }
-        wave_lds_sync();
-        // vertical DCT-II (q0, q1) / DST-II (q2, q3) on column li
-        dct2_8(c0);
-        dct2_8(c1);
-        dst2_8(c2);
-        dst2_8(c3);
-        // unfold, :2950-3008.  Lane li is column j; q1, q3 come from column j-1.
-        float p1[8], p3[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            p1[i] = lane_prev(c1[i]);
-            p3[i] = lane_prev(c3[i]);
-        }
-        // stage the full 15x15 unfold of each unit in its LDS block (fixed
-        // layout, centre (7, 7)); the wavefront then stores the (2r+1)^2
-        // window of its 8 units with contiguous stores
-        {
-            const int j = li;
-            const float s4 = 0.25f;
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const float a0 = c0[i];
-                const float a1 = j ? p1[i] : 0.f;
-                const float a2 = i ? c2[i - 1] : 0.f;
-                const float a3 = (i && j) ? p3[i - 1] : 0.f;
-                const float t1 = a0 + a2, t2 = a0 - a2, u1 = a1 + a3, u2 = a1 - a3;
-                L[(7 + i) * 15 + 7 + j] = s4 * (t1 + u1);
-                L[(7 + i) * 15 + 7 - j] = s4 * (t1 - u1);
-                L[(7 - i) * 15 + 7 + j] = s4 * (t2 + u2);
-                L[(7 - i) * 15 + 7 - j] = s4 * (t2 - u2);
-            }
-        }
+        corr_finish(c0, c1, c2, c3, L, li, fz2);
         wave_lds_sync();
         const int vflag = valid ? 1 : 0;
         for (int u = 0; u < 8; u++) {

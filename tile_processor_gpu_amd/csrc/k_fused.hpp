@@ -1,0 +1,220 @@
+// k_fused.hpp - forward CLT + pairwise correlation in one pass over the tiles.
+//
+// The staged path (convert_correct_tiles, then correlate2D) writes the whole
+// CLT (12 KB per tile) to memory and reads it back three times per camera.
+// Here a workgroup converts a group of GT tiles (4 cameras x 3 colors) into
+// LDS and correlates the group's pairs from there; storing the CLT to memory
+// is optional.  Same per-tile arithmetic as the staged kernels
+// (convert_unit, corr_accumulate, corr_finish), so results are identical.
+//
+// Workgroup: 3 wavefronts (one per color) of 8 units x 8 lanes, as in
+// k_convert_correct_tiles.  Per group of GT tiles:
+//   1. convert: each wavefront converts its color for GT x 4 (tile, camera)
+//      units (GT / 2 passes of 8 units) into clt_lds[GT][4][3][256];
+//   2. barrier; correlate: 6 GT (tile, pair) slots, 8 per wavefront, each
+//      accumulates its bicomplex product from LDS;
+//   3. barrier; the CLT region is reused as per-slot scratch for the
+//      normalisation, DCT-II back-transform and unfold; rows are stored;
+//   4. barrier before the next group overwrites LDS.
+//
+// Work items are runs of tiles_per_item consecutive tiles (in the visiting
+// order of swizzle_strip) of one frame.  Single-frame launches map one
+// workgroup to one item.  Stream launches (QUEUE) are persistent: a fixed
+// grid of workgroups takes items from a global counter until every frame's
+// items are done; frames are located by binary search on item_start.
+#pragma once
+
+#include "k_convert.hpp"
+#include "k_correlate.hpp"
+
+namespace tp {
+
+// Binary-identical to tp_frame (include/tp.h).
+struct FrameDesc {
+    const tp_task *tasks;
+    const float *const *images;
+    const float *const *kernels;
+    const float *const *koffs;
+    float *const *clt;  // may be null: CLT not stored
+    float *corrs;       // row 6 * task + pair
+    int num_tiles;
+    int item_start;
+};
+
+struct FusedArgs {
+    Geo g;
+    FrameDesc f0;              // single-frame launch
+    const FrameDesc *frames;   // stream launch: device array [nframes]
+    int nframes;
+    int *work;                 // stream launch: [0] next item, [1] number of items
+    unsigned long long *stamps;  // stream launch: per frame {first start, last end} or null
+    int tiles_per_item;        // multiple of GT
+    size_t dstride;
+    int lpf_mask;
+    float scale[3];
+    float fat_zero;
+    size_t corr_stride;
+    int radius;
+};
+
+template <int GT, bool QUEUE, int COLORS>
+__global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
+    static_assert(GT == 2 || GT == 4, "tiles per group");
+    constexpr int NSLOT = 6 * GT;
+    __shared__ __align__(16) float clt_lds[GT * 4 * 3 * 256];
+    __shared__ __align__(16) float cvt_lds[3][2][8][72];
+    __shared__ int s_item;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int unit = lane >> 3;
+    const int li = lane & 7;
+    const int cam = unit & 3;
+    float *lds_c = &cvt_lds[wave][0][unit][0];
+    float *lds_s = &cvt_lds[wave][1][unit][0];
+    const LaneFold lf = lane_fold(li);
+    int src_off[4];
+    corr_out_offsets(lane, a.radius, src_off);
+    const float fz2 = a.fat_zero * a.fat_zero;
+    const Geo &g = a.g;
+    const int total = QUEUE ? a.work[1] : (int)gridDim.x;
+    int item = QUEUE ? 0 : xcd_remap(blockIdx.x, gridDim.x);
+
+    while (true) {
+        if (QUEUE) {
+            if (threadIdx.x == 0) s_item = atomicAdd(&a.work[0], 1);
+            __syncthreads();
+            item = s_item;
+            __syncthreads();  // s_item is rewritten by the next iteration
+            if (item >= total) break;
+        }
+        int f = 0;
+        if (QUEUE) {  // last frame with item_start <= item
+            int lo = 0, hi = a.nframes - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.frames[mid].item_start <= item) lo = mid; else hi = mid - 1;
+            }
+            f = lo;
+        }
+        const FrameDesc fr = QUEUE ? a.frames[f] : a.f0;
+        if (QUEUE && a.stamps && threadIdx.x == 0)
+            atomicMin(&a.stamps[2 * f], (unsigned long long)wall_clock64());
+        ConvertArgs ca;
+        ca.g = g;
+        ca.koffs = fr.koffs;
+        ca.kernels = fr.kernels;
+        ca.images = fr.images;
+        ca.tasks = fr.tasks;
+        ca.clt = fr.clt;
+        ca.dstride = a.dstride;
+        ca.num_tiles = fr.num_tiles;
+        ca.lpf_mask = a.lpf_mask;
+        ca.pairs_per_block = 0;
+        const int s0 = (item - (QUEUE ? fr.item_start : 0)) * a.tiles_per_item;
+        const int s1 = min(fr.num_tiles, s0 + a.tiles_per_item);
+
+        for (int gs = s0; gs < s1; gs += GT) {
+            // (1) forward CLT of the group into LDS
+#pragma unroll
+            for (int pass = 0; pass < GT / 2; pass++) {
+                const int tl = pass * 2 + (unit >> 2);
+                const int s = gs + tl;
+                bool valid = s < s1;
+                const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
+                valid = valid && fr.tasks[task_num].task != 0;
+                float q0[8], q1[8], q2[8], q3[8];
+                int color;
+                if (wave == 0) {
+                    convert_unit<0>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    color = 0;
+                } else if (wave == 1) {
+                    convert_unit<1>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    color = 1;
+                } else {
+                    convert_unit<2>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    color = 2;
+                }
+                float *d = clt_lds + ((tl * 4 + cam) * 3 + color) * 256 + li * 8;
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) {
+                    const float *q = qq == 0 ? q0 : qq == 1 ? q1 : qq == 2 ? q2 : q3;
+                    *(float4 *)(d + qq * 64) = make_float4(q[0], q[1], q[2], q[3]);
+                    *(float4 *)(d + qq * 64 + 4) = make_float4(q[4], q[5], q[6], q[7]);
+                }
+                if (fr.clt) {
+                    const int tile = task_tile(g, fr.tasks[task_num].txy);
+                    if (valid && tile >= 0)
+                        store_clt_rows(fr.clt[cam] + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
+                                       q0, q1, q2, q3);
+                }
+            }
+            __syncthreads();
+
+            // (2) correlation products from LDS
+            const int slot = wave * 8 + unit;
+            const bool active = wave * 8 < NSLOT;  // wavefront-uniform
+            int row = -1;
+            float c0[8], c1[8], c2[8], c3[8];
+            if (active) {
+                const int tl = slot < NSLOT ? slot / 6 : 0;
+                const int pair = slot < NSLOT ? slot - 6 * tl : 0;
+                const int s = gs + tl;
+                if (slot < NSLOT && s < s1) {
+                    const int task_num = swizzle_strip(s, g.TX, fr.num_tiles);
+                    const tp_task *tk = fr.tasks + task_num;
+                    if (((tk->task >> (TP_TASK_CORR_BITS + pair)) & 1) && task_tile(g, tk->txy) >= 0)
+                        row = 6 * task_num + pair;
+                }
+                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * 256 + li * 8;
+                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * 256 + li * 8;
+                auto load = [&](int color, int which, int qq, int h) -> f32x4 {
+                    return ((const f32x4 *)((which ? tb : ta) + color * 256))[qq * 16 + h];
+                };
+                corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
+            }
+            __syncthreads();
+
+            // (3) normalise, back-transform, unfold in the (now free) CLT region
+            if (active) {
+                float *L = clt_lds + slot * 288;
+                corr_finish(c0, c1, c2, c3, L, li, fz2);
+                wave_lds_sync();
+                for (int u = 0; u < 8; u++) {
+                    const int r = __shfl(row, u * 8);
+                    if (r < 0) continue;
+                    const float *src = clt_lds + (wave * 8 + u) * 288;
+                    float *dst = fr.corrs + (size_t)r * a.corr_stride;
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (src_off[t] >= 0) store1(dst + lane + 64 * t, src[src_off[t]]);
+                }
+            }
+            __syncthreads();
+        }
+        if (QUEUE && a.stamps && threadIdx.x == 0)
+            atomicMax(&a.stamps[2 * f + 1], (unsigned long long)wall_clock64());
+        if (!QUEUE) break;
+    }
+}
+
+// Stream set-up: item_start of every frame, the queue counter and the
+// per-frame time stamps.  One wavefront; frames are few.
+__global__ void k_stream_setup(FrameDesc *frames, int nframes, int tiles_per_item, int *work,
+                               unsigned long long *stamps) {
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int f = 0; f < nframes; f++) {
+            frames[f].item_start = acc;
+            acc += (max(frames[f].num_tiles, 0) + tiles_per_item - 1) / tiles_per_item;
+        }
+        work[0] = 0;
+        work[1] = acc;
+    }
+    if (stamps)
+        for (int i = threadIdx.x; i < nframes; i += blockDim.x) {
+            stamps[2 * i] = ~0ull;
+            stamps[2 * i + 1] = 0ull;
+        }
+}
+
+}  // namespace tp

@@ -4,6 +4,7 @@
 // so the __constant__ tables are shared without relocatable device code.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdio>
 #include <algorithm>
 #include <string>
@@ -14,6 +15,7 @@
 #include "k_imclt.hpp"
 #include "k_texture.hpp"
 #include "k_combine.hpp"
+#include "k_fused.hpp"
 
 namespace {
 
@@ -47,6 +49,99 @@ int check_geometry(const tp_geometry *g) {
 }
 
 inline hipStream_t as_stream(tp_stream_t s) { return (hipStream_t)s; }
+
+static_assert(sizeof(tp_frame) == sizeof(tp::FrameDesc), "tp_frame layout");
+static_assert(offsetof(tp_frame, item_start) == offsetof(tp::FrameDesc, item_start), "tp_frame layout");
+
+struct FusedCfg {
+    int tpi, gt, bpc, dct;
+};
+
+int fused_config(const char *fn, const tp_fused_config *c, FusedCfg &o) {
+    o = {8, 2, 4, 0};
+    if (c) o = {c->tiles_per_item, c->tiles_per_group, c->blocks_per_cu, c->dct};
+    if (o.gt != 2 && o.gt != 4) return fail(TP_EINVAL, "%s: tiles_per_group must be 2 or 4", fn);
+    if (o.tpi < o.gt || o.tpi % o.gt || o.tpi > 4096)
+        return fail(TP_EINVAL, "%s: tiles_per_item must be a multiple of tiles_per_group, <= 4096", fn);
+    if (o.bpc < 1 || o.bpc > 8) return fail(TP_EINVAL, "%s: blocks_per_cu must be 1..8", fn);
+    if (o.dct != 0) return fail(TP_EINVAL, "%s: dct must be 0", fn);
+    return TP_OK;
+}
+
+int fused_common(const char *fn, const tp_geometry *geom, size_t dstride, int lpf_mask, int colors,
+                 size_t corr_stride, int corr_radius) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (dstride < (size_t)geom->img_width) return fail(TP_EINVAL, "%s: dstride < image width", fn);
+    if (lpf_mask & ~7) return fail(TP_EINVAL, "%s: lpf_mask has bits above 2", fn);
+    if (colors != 1 && colors != 3) return fail(TP_EINVAL, "%s: colors must be 1 or 3", fn);
+    if (corr_radius < 0 || corr_radius > TP_MAX_CORR_RADIUS)
+        return fail(TP_EINVAL, "%s: corr_radius must be 0..7", fn);
+    const size_t n = 2 * corr_radius + 1;
+    if (corr_stride < n * n) return fail(TP_EINVAL, "%s: corr_stride < (2r+1)^2", fn);
+    return TP_OK;
+}
+
+tp::FusedArgs fused_args(const tp_geometry *geom, size_t dstride, int lpf_mask, float s0, float s1,
+                         float s2, float fat_zero, size_t corr_stride, int corr_radius, int tpi) {
+    tp::FusedArgs a = {};
+    a.g = tp::make_geo(*geom);
+    a.tiles_per_item = tpi;
+    a.dstride = dstride;
+    a.lpf_mask = lpf_mask;
+    a.scale[0] = s0;
+    a.scale[1] = s1;
+    a.scale[2] = s2;
+    a.fat_zero = fat_zero;
+    a.corr_stride = corr_stride;
+    a.radius = corr_radius;
+    return a;
+}
+
+template <bool Q>
+void launch_fused(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
+    const dim3 gd(grid), bd(192);
+    if (c.gt == 2) {
+        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<2, Q, 3>), gd, bd, 0, st, a);
+        else hipLaunchKernelGGL((tp::k_fused<2, Q, 1>), gd, bd, 0, st, a);
+    } else {
+        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<4, Q, 3>), gd, bd, 0, st, a);
+        else hipLaunchKernelGGL((tp::k_fused<4, Q, 1>), gd, bd, 0, st, a);
+    }
+}
+
+int num_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    return n;
+}
+
+// Enqueue the stream form (set-up kernel + persistent kernel) on st.
+int enqueue_stream(const tp_geometry *geom, tp_frame *gpu_frames, int nframes, int32_t *gpu_work,
+                   uint64_t *gpu_stamps, size_t dstride, int lpf_mask, int colors, float s0,
+                   float s1, float s2, float fat_zero, size_t corr_stride, int corr_radius,
+                   const tp_fused_config *config, hipStream_t st, const char *fn) {
+    int rc = fused_common(fn, geom, dstride, lpf_mask, colors, corr_stride, corr_radius);
+    if (rc) return rc;
+    if (!gpu_frames || !gpu_work) return fail(TP_EINVAL, "%s: NULL pointer argument", fn);
+    if (nframes <= 0) return fail(TP_EINVAL, "%s: nframes must be positive", fn);
+    FusedCfg c;
+    if ((rc = fused_config(fn, config, c))) return rc;
+    const int cus = num_cus();
+    if (cus <= 0) return fail(TP_EHIP, "%s: cannot query the device", fn);
+    tp::FusedArgs a = fused_args(geom, dstride, lpf_mask, s0, s1, s2, fat_zero, corr_stride,
+                                 corr_radius, c.tpi);
+    a.frames = (const tp::FrameDesc *)gpu_frames;
+    a.nframes = nframes;
+    a.work = gpu_work;
+    a.stamps = (unsigned long long *)gpu_stamps;
+    hipLaunchKernelGGL(tp::k_stream_setup, dim3(1), dim3(64), 0, st, (tp::FrameDesc *)gpu_frames,
+                       nframes, c.tpi, gpu_work, a.stamps);
+    if ((rc = check_launch("stream_setup"))) return rc;
+    launch_fused<true>(a, c, colors, cus * c.bpc, st);
+    return check_launch("fused (stream)");
+}
 
 }  // namespace
 
@@ -411,6 +506,105 @@ int tp_corr_disparity(const float *gpu_tiles, size_t stride, int num_tiles, int 
     hipLaunchKernelGGL(tp::k_corr_disparity, dim3((num_tiles + 255) / 256), dim3(256), 0,
                        as_stream(stream), gpu_tiles, stride, num_tiles, corr_radius, mode, gpu_out);
     return check_launch("corr_disparity");
+}
+
+int tp_convert_correlate(const tp_geometry *geom, float **gpu_kernel_offsets, float **gpu_kernels,
+                         float **gpu_images, const tp_task *gpu_tasks, float **gpu_clt,
+                         size_t dstride, int num_tiles, int lpf_mask, int colors, float scale0,
+                         float scale1, float scale2, float fat_zero, size_t corr_stride,
+                         int corr_radius, float *gpu_corrs, const tp_fused_config *config,
+                         tp_stream_t stream) {
+    const char *fn = "tp_convert_correlate";
+    int rc = fused_common(fn, geom, dstride, lpf_mask, colors, corr_stride, corr_radius);
+    if (rc) return rc;
+    if (!gpu_kernel_offsets || !gpu_kernels || !gpu_images || !gpu_tasks || !gpu_corrs)
+        return fail(TP_EINVAL, "%s: NULL pointer argument", fn);
+    if (num_tiles < 0) return fail(TP_EINVAL, "%s: num_tiles < 0", fn);
+    FusedCfg c;
+    if ((rc = fused_config(fn, config, c))) return rc;
+    g_err.clear();
+    if (num_tiles == 0) return TP_OK;
+    tp::FusedArgs a = fused_args(geom, dstride, lpf_mask, scale0, scale1, scale2, fat_zero,
+                                 corr_stride, corr_radius, c.tpi);
+    a.f0.tasks = gpu_tasks;
+    a.f0.images = (const float *const *)gpu_images;
+    a.f0.kernels = (const float *const *)gpu_kernels;
+    a.f0.koffs = (const float *const *)gpu_kernel_offsets;
+    a.f0.clt = gpu_clt;
+    a.f0.corrs = gpu_corrs;
+    a.f0.num_tiles = num_tiles;
+    a.f0.item_start = 0;
+    launch_fused<false>(a, c, colors, (num_tiles + c.tpi - 1) / c.tpi, as_stream(stream));
+    return check_launch("fused");
+}
+
+int tp_stream_convert_correlate(const tp_geometry *geom, tp_frame *gpu_frames, int nframes,
+                                int32_t *gpu_work, uint64_t *gpu_stamps, size_t dstride,
+                                int lpf_mask, int colors, float scale0, float scale1, float scale2,
+                                float fat_zero, size_t corr_stride, int corr_radius,
+                                const tp_fused_config *config, tp_stream_t stream) {
+    g_err.clear();
+    return enqueue_stream(geom, gpu_frames, nframes, gpu_work, gpu_stamps, dstride, lpf_mask, colors,
+                          scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, config,
+                          as_stream(stream), "tp_stream_convert_correlate");
+}
+
+int tp_stream_graph_create(const tp_geometry *geom, tp_frame *gpu_frames, int nframes,
+                           int32_t *gpu_work, uint64_t *gpu_stamps, size_t dstride, int lpf_mask,
+                           int colors, float scale0, float scale1, float scale2, float fat_zero,
+                           size_t corr_stride, int corr_radius, const tp_fused_config *config,
+                           tp_graph_t *graph) {
+    const char *fn = "tp_stream_graph_create";
+    if (!graph) return fail(TP_EINVAL, "%s: graph is NULL", fn);
+    *graph = nullptr;
+    g_err.clear();
+    hipStream_t st;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_stream_graph_create: %s", hipGetErrorString(e));
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        return fail(TP_EHIP, "tp_stream_graph_create: begin capture: %s", hipGetErrorString(e));
+    }
+    int rc = enqueue_stream(geom, gpu_frames, nframes, gpu_work, gpu_stamps, dstride, lpf_mask, colors,
+                            scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, config, st, fn);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    (void)hipStreamDestroy(st);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_stream_graph_create: end capture: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_stream_graph_create: instantiate: %s", hipGetErrorString(e));
+    *graph = (tp_graph_t)ex;
+    return TP_OK;
+}
+
+int tp_graph_launch(tp_graph_t graph, tp_stream_t stream) {
+    if (!graph) return fail(TP_EINVAL, "tp_graph_launch: graph is NULL%s", "");
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, as_stream(stream));
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_graph_launch: %s", hipGetErrorString(e));
+    g_err.clear();
+    return TP_OK;
+}
+
+int tp_graph_destroy(tp_graph_t graph) {
+    if (!graph) return TP_OK;
+    hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_graph_destroy: %s", hipGetErrorString(e));
+    g_err.clear();
+    return TP_OK;
+}
+
+double tp_wall_clock_hz(void) {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0.0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0.0;
+    return 1e3 * khz;
 }
 
 }  // extern "C"

@@ -140,3 +140,38 @@ def config_full(seed=2):
     xy = tile_centres(geom, d, 0.5, rng)
     return dict(geom=geom, images=images, kernels=kern, koffs=koffs, tasks=full_tasks(geom, xy),
                 disparity=d)
+
+
+def cluster_mask(geom, rng, frac, sigma=3.0):
+    """Clustered tile mask: Gaussian-smoothed white noise over the tile grid,
+    thresholded so that a fraction frac of the tiles is active."""
+    noise = _blur(rng.standard_normal((geom.tilesy, geom.tilesx)), sigma)
+    k = int(round(frac * geom.num_tiles))
+    if k <= 0:
+        return np.zeros(geom.num_tiles, bool)
+    thr = np.sort(noise.ravel())[::-1][k - 1]
+    return (noise.ravel() >= thr)
+
+
+def sparse_tasks(tasks, mask):
+    """Task list of the active tiles only (list order = tile order)."""
+    return np.ascontiguousarray(tasks[np.asarray(mask, bool)])
+
+
+def config_stream(nframes=64, seed=4, geom=None, frac=(0.1, 1.0)):
+    """Stream of small-config frames (4 x 256 x 256), each with its own
+    images, isotropic PSF kernels (sigma 0.5-1.5), fractional shifts
+    U[-0.5, 0.5] and a clustered sparse task list whose active fraction is
+    uniform over frac."""
+    geom = geom or Geometry(256, 256)
+    rng = np.random.default_rng(seed)
+    frames = []
+    for _ in range(nframes):
+        images = make_images(geom, rng)
+        kern, koffs = make_kernels(geom, rng, sigma=(0.5, 1.5))
+        xy = tile_centres(geom, 0.0, 0.5, rng)
+        f = rng.uniform(*frac)
+        tasks = sparse_tasks(full_tasks(geom, xy), cluster_mask(geom, rng, f))
+        frames.append(dict(geom=geom, images=images, kernels=kern, koffs=koffs, tasks=tasks,
+                           active=f))
+    return frames

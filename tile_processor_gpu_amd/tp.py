@@ -99,6 +99,127 @@ def imclt_rbg_all(geom, gpu_clt, gpu_rbg, dstride, stream=None):
               _stream(stream))
 
 
+class FusedConfig(ctypes.Structure):
+    """tp_fused_config: work-item size, tiles held in LDS at once, resident
+    workgroups per CU (stream form) and the forward-DCT method."""
+    _fields_ = [("tiles_per_item", ctypes.c_int32), ("tiles_per_group", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("dct", ctypes.c_int32)]
+
+    def __init__(self, tiles_per_item=8, tiles_per_group=2, blocks_per_cu=4, dct=0):
+        super().__init__(tiles_per_item, tiles_per_group, blocks_per_cu, dct)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _cfg(config):
+    if config is None:
+        return _VP(0)
+    if isinstance(config, dict):
+        config = FusedConfig(**config)
+    return ctypes.byref(config)
+
+
+def convert_correlate(geom, gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt,
+                      dstride, num_tiles, lpf_mask, colors, scale0, scale1, scale2, fat_zero,
+                      corr_stride, corr_radius, gpu_corrs, config=None, stream=None):
+    """Forward CLT + pair correlation of one frame in one pass (rows 6 task + pair);
+    gpu_clt may be None."""
+    _check_dev(gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt, gpu_corrs)
+    g = geom.c()
+    _lib.call("tp_convert_correlate", ctypes.byref(g), _p(gpu_kernel_offsets), _p(gpu_kernels),
+              _p(gpu_images), _p(gpu_tasks), _p(gpu_clt), int(dstride), int(num_tiles),
+              int(lpf_mask), int(colors), float(scale0), float(scale1), float(scale2),
+              float(fat_zero), int(corr_stride), int(corr_radius), _p(gpu_corrs), _cfg(config),
+              _stream(stream))
+
+
+# tp_frame (include/tp.h): one frame of the stream form
+FRAME_DTYPE = np.dtype([("tasks", "<u8"), ("images", "<u8"), ("kernels", "<u8"),
+                        ("kernel_offsets", "<u8"), ("clt", "<u8"), ("corrs", "<u8"),
+                        ("num_tiles", "<i4"), ("item_start", "<i4")])
+
+
+def wall_clock_hz():
+    return _lib.lib().tp_wall_clock_hz()
+
+
+class StreamRun:
+    """The stream form over a list of FrameBuffers: one persistent launch
+    (tp_stream_convert_correlate) per call, or a captured graph of it."""
+
+    def __init__(self, frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
+                 store_clt=False, config=None):
+        if not frames:
+            raise ValueError("no frames")
+        self.frames = frames
+        geom = frames[0].geom
+        dev = frames[0].device
+        for fb in frames:
+            if fb.geom != geom or fb.dstride != frames[0].dstride or \
+                    fb.corr_radius != frames[0].corr_radius or \
+                    fb.corr_stride != frames[0].corr_stride:
+                raise ValueError("frames of one stream share geometry, dstride and correlation shape")
+        self.geom, self.device = geom, dev
+        self.args = (colors, *scales, fat_zero)
+        self.lpf_mask = lpf_mask
+        self.config = FusedConfig(**config) if isinstance(config, dict) else (config or FusedConfig())
+        desc = np.zeros(len(frames), dtype=FRAME_DTYPE)
+        for i, fb in enumerate(frames):
+            desc[i] = (fb.tasks.data_ptr(), fb.p_images.data_ptr(), fb.p_kernels.data_ptr(),
+                       fb.p_koffs.data_ptr(), fb.p_clt.data_ptr() if store_clt else 0,
+                       fb.corrs.data_ptr(), fb.num_tiles, 0)
+        self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+        self.work = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev)
+        self.graph = None
+
+    def _common(self):
+        f0 = self.frames[0]
+        colors, s0, s1, s2, fz = self.args
+        return (ctypes.byref(self.geom.c()), _p(self.desc), len(self.frames), _p(self.work),
+                _p(self.stamps), int(f0.dstride), int(self.lpf_mask), int(colors), float(s0),
+                float(s1), float(s2), float(fz), int(f0.corr_stride), int(f0.corr_radius),
+                ctypes.byref(self.config))
+
+    def launch(self, stream=None):
+        _lib.call("tp_stream_convert_correlate", *self._common(), _stream(stream))
+
+    def capture(self):
+        self.release()
+        h = _VP(0)
+        _lib.call("tp_stream_graph_create", *self._common(), ctypes.byref(h))
+        self.graph = h
+
+    def replay(self, stream=None):
+        if self.graph is None:
+            self.capture()
+        _lib.call("tp_graph_launch", self.graph, _stream(stream))
+
+    def release(self):
+        if self.graph is not None:
+            _lib.call("tp_graph_destroy", self.graph)
+            self.graph = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    def frame_times_s(self):
+        """Per frame (start, end) of its work items in seconds after the
+        first start, from the GPU wall clock (call after synchronising)."""
+        raw = self.stamps.cpu().numpy()
+        hz = wall_clock_hz()
+        done = raw[:, 0] != -1  # frames without work items keep the initial stamps
+        out = np.full(raw.shape, np.nan)
+        if done.any():
+            t0 = raw[done, 0].min()
+            out[done] = (raw[done] - t0).astype(np.float64) / hz
+        return out
+
+
 class FrameBuffers:
     """Device buffers of one 4-camera frame in the reference layout, with the
     pointer arrays the `float **` arguments need."""
@@ -161,6 +282,17 @@ class FrameBuffers:
         correlate2D(self.geom, self.p_clt, colors, scales[0], scales[1], scales[2], fat_zero,
                     self.num_corr, self.corr_idx, self.corr_stride, self.corr_radius, self.corrs,
                     stream)
+
+    def fused(self, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
+              store_clt=False, config=None, stream=None):
+        """Forward CLT + correlation in one pass; rows of self.corrs are
+        6 task + pair (self.corrs needs 6 num_tiles rows)."""
+        if self.corrs.shape[0] < 6 * self.num_tiles:
+            raise ValueError("corrs needs 6 rows per task for the fused form")
+        convert_correlate(self.geom, self.p_koffs, self.p_kernels, self.p_images, self.tasks,
+                          self.p_clt if store_clt else None, self.dstride, self.num_tiles,
+                          lpf_mask, colors, scales[0], scales[1], scales[2], fat_zero,
+                          self.corr_stride, self.corr_radius, self.corrs, config, stream)
 
 
 # texture parameters of the reference harness (src/dtt8x8.cu:791-814)
