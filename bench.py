@@ -122,6 +122,86 @@ def cpu_baseline(cfg, idx, rows, reps=3):
                       f"convert + correlate, FP32 oracle, best of {reps}: {dt:.3f} s"}
 
 
+def median_ms(fn, reps, warmup=3):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+
+def measure_fused(fb, reps, staged_ms, frame_bytes_fused):
+    """Fused forward CLT + correlation of the same frame (CLT kept on chip),
+    with the autotuned launch configuration; correlations checked against
+    the staged path's."""
+    import torch
+    from tile_processor_gpu_amd import autotune
+    ref = fb.corrs[:fb.num_corr].clone()
+    report = []
+    cfg = autotune.tune_fused(fb, 3, SCALES, FAT_ZERO, cache_path=None, report=report)
+    ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, config=cfg), reps)
+    ms_clt = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, store_clt=True, config=cfg), reps)
+    dev = float((fb.corrs[:fb.num_corr] - ref).abs().max() / ref.abs().max())
+    torch.cuda.synchronize()
+    return {"frames_per_s": round(1e3 / ms, 2), "ms": round(ms, 4),
+            "speedup_vs_plain": round(staged_ms / ms, 3),
+            "ms_with_clt_store": round(ms_clt, 4),
+            "hbm_GBps_algorithmic": round(frame_bytes_fused / (ms * 1e-3) / 1e9, 1),
+            "config": cfg, "max_rel_dev_vs_plain": dev,
+            "tuned_over": len(report)}
+
+
+def measure_stream(nframes, reps, seed=4):
+    """Batched stream: nframes small-config frames with clustered sparse task
+    lists, one persistent launch per batch replayed from a captured graph,
+    autotuned; per-frame latency from the kernel's GPU wall-clock stamps."""
+    import torch
+    from tile_processor_gpu_amd import autotune, synth, tp
+    from tile_processor_gpu_amd.geometry import corr_indices
+    frames = synth.config_stream(nframes, seed=seed)
+    fbs = []
+    for f in frames:
+        fb = tp.FrameBuffers(f["geom"], "cuda:0", corr_radius=RADIUS, max_corr=6 * len(f["tasks"]))
+        fb.upload(f["images"], f["kernels"], f["koffs"], f["tasks"], corr_indices(f["geom"], f["tasks"]))
+        fbs.append(fb)
+    report = []
+    cfg = autotune.tune_stream(fbs, 3, SCALES, FAT_ZERO, cache_path=None, report=report)
+    run = tp.StreamRun(fbs, 3, SCALES, FAT_ZERO, config=cfg)
+    run.capture()
+    ms = median_ms(run.replay, reps)
+    run.replay()
+    torch.cuda.synchronize()
+    t = run.frame_times_s()
+    service = (t[:, 1] - t[:, 0]) * 1e3
+    done = t[:, 1] * 1e3
+    tiles = sum(fb.num_tiles for fb in fbs)
+    # staged per-frame launches of the same stream, for comparison
+    def staged():
+        for fb in fbs:
+            fb.convert()
+            fb.correlate(3, SCALES, FAT_ZERO)
+    staged_ms = median_ms(staged, max(3, reps // 4), warmup=1)
+    run.release()
+    return {"frames": nframes, "tiles": int(tiles),
+            "active_fraction": [round(min(f["active"] for f in frames), 3),
+                                round(max(f["active"] for f in frames), 3)],
+            "frames_per_s": round(nframes * 1e3 / ms, 1), "ms_per_batch": round(ms, 4),
+            "staged_ms_per_batch": round(staged_ms, 4),
+            "speedup_vs_staged": round(staged_ms / ms, 3),
+            "latency_ms": {"service_p50": round(float(np.percentile(service, 50)), 4),
+                           "service_p99": round(float(np.percentile(service, 99)), 4),
+                           "completion_p50": round(float(np.percentile(done, 50)), 4),
+                           "completion_p99": round(float(np.percentile(done, 99)), 4)},
+            "config": cfg, "tuned_over": len(report), "graph": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +211,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=242, help="tile rows for the CPU baseline (0: skip)")
     ap.add_argument("--err-tiles", type=int, default=512,
                     help="tiles checked against the FP64 oracle after timing (0: skip)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the plain staged path (no fused / stream measurements)")
+    ap.add_argument("--stream-frames", type=int, default=64)
     ap.add_argument("--mode", choices=["frame", "copy"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -219,6 +302,14 @@ def main():
         err = err_vs_fp64(cfg, fb, idx, sample)
         log(f"err vs fp64: {err}")
 
+    extra = {}
+    if not args.no_extra:
+        fused_bytes = (conv_b - 4 * g.num_tiles * 3 * 1024) + (corr_b - 4 * g.num_tiles * 3 * 1024)
+        extra["fused"] = measure_fused(fb, max(5, K // 2), 1e3 * total_s / K, fused_bytes)
+        log(f"fused: {extra['fused']}")
+        extra["stream"] = measure_stream(args.stream_frames, max(5, K // 2))
+        log(f"stream: {extra['stream']}")
+
     cpu = None
     if args.cpu_rows > 0:
         cpu = cpu_baseline(cfg, idx, args.cpu_rows)
@@ -248,6 +339,7 @@ def main():
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 
