@@ -425,6 +425,52 @@ int tp_graph_destroy(tp_graph_t graph);
 /* Ticks per second of the clock behind gpu_stamps. */
 double tp_wall_clock_hz(void);
 
+/* ------------------------------------------------------------------------
+ * Disparity-refinement CNN (no counterpart in the reference: its Java host
+ * took a parabolic/centroid argmax of the correlation tiles).  Per tile, 7
+ * input planes of 15 x 15 (the combined correlation and the 6 pair tiles
+ * mapped onto the horizontal disparity axis), two 3x3 convolutions of 32
+ * channels with ReLU, a 3x3 convolution to a logit map and a confidence
+ * map; the residual disparity is the soft-argmax column of the logits
+ * (px from the centre), the confidence the sigmoid of the confidence map
+ * pooled by the same softmax.  Parameters: TP_NN_PARAMS values in the
+ * layout W1[32][7][3][3], b1[32], W2[32][32][3][3], b2[32], W3[2][32][3][3],
+ * b3[2]; the kernels read a BF16 copy and compute in FP32.
+ * ---------------------------------------------------------------------- */
+#define TP_NN_PARAMS 11874
+#define TP_NN_INPUT 1575          /* floats of input per tile (7 x 15 x 15) */
+#define TP_NN_ACTS 14850          /* floats of saved activations per tile */
+
+/* Network input from fused/stream correlation rows (6 task + pair, radius
+ * 7): gpu_x[num_tasks][7][15][15]. */
+int tp_nn_input(const tp_geometry *geom, const tp_task *gpu_tasks, int num_tasks,
+                const float *gpu_corrs, size_t corr_stride, float *gpu_x, tp_stream_t stream);
+
+/* gpu_out[n][2] = (residual disparity px, confidence); gpu_acts (NULL for
+ * inference) receives TP_NN_ACTS floats per tile for tp_nn_backward. */
+int tp_nn_forward(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_acts,
+                  float *gpu_out, tp_stream_t stream);
+
+/* Adds the gradient of the mean training loss over the tiles with a finite
+ * target (huber(d - target) + conf_weight * BCE(confidence, |d - target| <
+ * 0.5)) to gpu_grads[TP_NN_PARAMS] and the loss to gpu_loss[0];
+ * loss_scale = 1 / (number of tiles with a finite target). */
+int tp_nn_backward(const uint16_t *gpu_params_bf16, const float *gpu_x, const float *gpu_acts,
+                   const float *gpu_target, int n, float conf_weight, float huber_delta,
+                   float loss_scale, float *gpu_grads, float *gpu_loss, tp_stream_t stream);
+
+/* One Adam step on the FP32 master parameters (step >= 1), refreshing the
+ * BF16 copy. */
+int tp_nn_adam(float *gpu_params, float *gpu_m, float *gpu_v, const float *gpu_grads, int n,
+               float lr, float beta1, float beta2, float eps, int step,
+               uint16_t *gpu_params_bf16, tp_stream_t stream);
+
+/* BF16 copy (round to nearest even) of n FP32 values. */
+int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t stream);
+
+/* gpu_dst[0..n) = value, by a kernel (safe inside captured graphs). */
+int tp_fill(float *gpu_dst, size_t n, float value, tp_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -284,3 +284,46 @@ def corr_disparity(tiles, radius, mode=0):
                                     ctypes.c_int(tiles.shape[0]), ctypes.c_int(radius),
                                     ctypes.c_int(mode), out.ctypes.data_as(_VP))
     return out
+
+
+NN_PARAMS = 11874
+NN_ACTS = 14850
+
+
+def bf16_round(a):
+    """Round float32 values to the nearest BF16 (ties to even), as float32."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def nn_forward(prm, x, with_acts=False):
+    """prm: [11874] parameters (float64; pass the BF16-rounded values to
+    mirror the GPU); x: [n][7][225] -> out [n][2] (disparity, confidence)
+    and optionally acts [n][14850]."""
+    prm = np.ascontiguousarray(prm, np.float64)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 7 * 225)
+    n = x.shape[0]
+    out = np.zeros((n, 2))
+    acts = np.zeros((n, NN_ACTS)) if with_acts else None
+    lib().oracle_nn_forward(prm.ctypes.data_as(_VP), x.ctypes.data_as(_VP), ctypes.c_int(n),
+                            acts.ctypes.data_as(_VP) if with_acts else _VP(0), out.ctypes.data_as(_VP))
+    return (out, acts) if with_acts else out
+
+
+def nn_backward(prm, x, target, conf_weight=0.1, delta=1.0, scale=None):
+    """Gradient (float64[11874]) and value of the mean training loss over the
+    tiles with a finite target."""
+    prm = np.ascontiguousarray(prm, np.float64)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 7 * 225)
+    target = np.ascontiguousarray(target, np.float32)
+    n = x.shape[0]
+    if scale is None:
+        scale = 1.0 / max(1, int(np.isfinite(target).sum()))
+    g = np.zeros(NN_PARAMS)
+    loss = np.zeros(1)
+    lib().oracle_nn_backward(prm.ctypes.data_as(_VP), x.ctypes.data_as(_VP), ctypes.c_int(n),
+                             target.ctypes.data_as(_VP), ctypes.c_double(conf_weight),
+                             ctypes.c_double(delta), ctypes.c_double(scale), g.ctypes.data_as(_VP),
+                             loss.ctypes.data_as(_VP))
+    return g, float(loss[0])

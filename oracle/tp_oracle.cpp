@@ -1117,4 +1117,150 @@ void oracle_corr_disparity_f64(const double *tiles, size_t stride, int ntiles, i
     for (int t = 0; t < ntiles; t++) corr_disparity<double>(tiles + (size_t)t * stride, radius, mode, out + 2 * t);
 }
 
+/* ------------------------------------------------------------------------
+ * Disparity-refinement CNN (not in the reference).  Plain loops in double;
+ * the parameters are doubles (the tests pass the BF16-rounded values the GPU
+ * kernels read).  Layout: W1[32][7][9], b1[32], W2[32][32][9], b2[32],
+ * W3[2][32][9], b3[2]; tiles 15 x 15 with zero padding.
+ * ---------------------------------------------------------------------- */
+static const int NN_P = 225;
+
+static void nn_conv(const double *in, int cin, const double *w, const double *b, int cout, bool relu,
+                    double *out) {
+    for (int co = 0; co < cout; co++)
+        for (int y = 0; y < 15; y++)
+            for (int x = 0; x < 15; x++) {
+                double acc = b[co];
+                for (int ci = 0; ci < cin; ci++)
+                    for (int r = 0; r < 9; r++) {
+                        const int yy = y + r / 3 - 1, xx = x + r % 3 - 1;
+                        if (yy < 0 || yy >= 15 || xx < 0 || xx >= 15) continue;
+                        acc += (double)w[(co * cin + ci) * 9 + r] * in[ci * NN_P + yy * 15 + xx];
+                    }
+                out[co * NN_P + y * 15 + x] = relu ? std::max(acc, 0.0) : acc;
+            }
+}
+
+/* softmax over z0, disparity = sum p (x - 7), confidence logit = sum p z1 */
+static void nn_head(const double *z, double *prob, double &disp, double &cl) {
+    double mx = -1e300;
+    for (int i = 0; i < NN_P; i++) mx = std::max(mx, z[i]);
+    double s = 0, sx = 0, sc = 0;
+    for (int i = 0; i < NN_P; i++) {
+        prob[i] = std::exp(z[i] - mx);
+        s += prob[i];
+    }
+    for (int i = 0; i < NN_P; i++) {
+        prob[i] /= s;
+        sx += prob[i] * (i % 15 - 7);
+        sc += prob[i] * z[NN_P + i];
+    }
+    disp = sx;
+    cl = sc;
+}
+
+/* out[n][2] = (disparity, confidence); acts[n][14850] = h1, h2, z (or NULL) */
+void oracle_nn_forward(const double *prm, const float *x, int n, double *acts, double *out) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int t = 0; t < n; t++) {
+        std::vector<double> in(7 * NN_P), h1(32 * NN_P), h2(32 * NN_P), z(2 * NN_P), prob(NN_P);
+        for (int i = 0; i < 7 * NN_P; i++) in[i] = x[(size_t)t * 7 * NN_P + i];
+        nn_conv(in.data(), 7, prm + 0, prm + 2016, 32, true, h1.data());
+        nn_conv(h1.data(), 32, prm + 2048, prm + 11264, 32, true, h2.data());
+        nn_conv(h2.data(), 32, prm + 11296, prm + 11872, 2, false, z.data());
+        double disp, cl;
+        nn_head(z.data(), prob.data(), disp, cl);
+        out[2 * t] = disp;
+        out[2 * t + 1] = 1.0 / (1.0 + std::exp(-cl));
+        if (acts) {
+            double *a = acts + (size_t)t * 14850;
+            std::copy(h1.begin(), h1.end(), a);
+            std::copy(h2.begin(), h2.end(), a + 32 * NN_P);
+            std::copy(z.begin(), z.end(), a + 64 * NN_P);
+        }
+    }
+}
+
+/* Gradient of sum over tiles with a finite target of scale * (huber_delta(d
+ * - target) + conf_weight * BCE(sigmoid(cl), |d - target| < 0.5)), the BCE
+ * label held constant; added to grads[11874], loss added to loss[0]. */
+void oracle_nn_backward(const double *prm, const float *x, int n, const float *target,
+                        double conf_weight, double delta, double scale, double *grads, double *loss) {
+    const double *W2 = prm + 2048, *W3 = prm + 11296;
+    for (int t = 0; t < n; t++) {
+        if (!std::isfinite(target[t])) continue;
+        std::vector<double> in(7 * NN_P), h1(32 * NN_P), h2(32 * NN_P), z(2 * NN_P), prob(NN_P);
+        for (int i = 0; i < 7 * NN_P; i++) in[i] = x[(size_t)t * 7 * NN_P + i];
+        nn_conv(in.data(), 7, prm + 0, prm + 2016, 32, true, h1.data());
+        nn_conv(h1.data(), 32, W2, prm + 11264, 32, true, h2.data());
+        nn_conv(h2.data(), 32, W3, prm + 11872, 2, false, z.data());
+        double disp, cl;
+        nn_head(z.data(), prob.data(), disp, cl);
+        const double e = disp - target[t];
+        const double hd = std::min(std::max(e, -delta), delta);
+        const double sg = 1.0 / (1.0 + std::exp(-cl));
+        const double lab = std::fabs(e) < 0.5 ? 1.0 : 0.0;
+        const double gc = conf_weight * (sg - lab);
+        const double ae = std::fabs(e);
+        const double hub = ae <= delta ? 0.5 * e * e : delta * (ae - 0.5 * delta);
+        const double bce = -(lab * std::log(std::max(sg, 1e-12)) + (1 - lab) * std::log(std::max(1 - sg, 1e-12)));
+        loss[0] += scale * (hub + conf_weight * bce);
+        // dz
+        std::vector<double> dz(2 * NN_P), dh2(32 * NN_P, 0.0), dh1(32 * NN_P, 0.0);
+        for (int i = 0; i < NN_P; i++) {
+            dz[i] = scale * prob[i] * (hd * ((i % 15 - 7) - disp) + gc * (z[NN_P + i] - cl));
+            dz[NN_P + i] = scale * gc * prob[i];
+        }
+        // layer 3
+        for (int co = 0; co < 2; co++)
+            for (int p = 0; p < NN_P; p++) {
+                const double d = dz[co * NN_P + p];
+                grads[11872 + co] += d;
+                const int y = p / 15, xx0 = p % 15;
+                for (int ci = 0; ci < 32; ci++)
+                    for (int r = 0; r < 9; r++) {
+                        const int yy = y + r / 3 - 1, xx = xx0 + r % 3 - 1;
+                        if (yy < 0 || yy >= 15 || xx < 0 || xx >= 15) continue;
+                        const int q = ci * NN_P + yy * 15 + xx;
+                        grads[11296 + (co * 32 + ci) * 9 + r] += d * h2[q];
+                        dh2[q] += d * W3[(co * 32 + ci) * 9 + r];
+                    }
+            }
+        for (int i = 0; i < 32 * NN_P; i++)
+            if (!(h2[i] > 0)) dh2[i] = 0;
+        // layer 2
+        for (int co = 0; co < 32; co++)
+            for (int p = 0; p < NN_P; p++) {
+                const double d = dh2[co * NN_P + p];
+                if (d == 0) continue;
+                grads[11264 + co] += d;
+                const int y = p / 15, xx0 = p % 15;
+                for (int ci = 0; ci < 32; ci++)
+                    for (int r = 0; r < 9; r++) {
+                        const int yy = y + r / 3 - 1, xx = xx0 + r % 3 - 1;
+                        if (yy < 0 || yy >= 15 || xx < 0 || xx >= 15) continue;
+                        const int q = ci * NN_P + yy * 15 + xx;
+                        grads[2048 + (co * 32 + ci) * 9 + r] += d * h1[q];
+                        dh1[q] += d * W2[(co * 32 + ci) * 9 + r];
+                    }
+            }
+        for (int i = 0; i < 32 * NN_P; i++)
+            if (!(h1[i] > 0)) dh1[i] = 0;
+        // layer 1
+        for (int co = 0; co < 32; co++)
+            for (int p = 0; p < NN_P; p++) {
+                const double d = dh1[co * NN_P + p];
+                if (d == 0) continue;
+                grads[2016 + co] += d;
+                const int y = p / 15, xx0 = p % 15;
+                for (int ci = 0; ci < 7; ci++)
+                    for (int r = 0; r < 9; r++) {
+                        const int yy = y + r / 3 - 1, xx = xx0 + r % 3 - 1;
+                        if (yy < 0 || yy >= 15 || xx < 0 || xx >= 15) continue;
+                        grads[(co * 7 + ci) * 9 + r] += d * in[ci * NN_P + yy * 15 + xx];
+                    }
+            }
+    }
+}
+
 }  // extern "C"

@@ -42,6 +42,12 @@ PROTOTYPES = {
     "tp_graph_launch": [_P, _P],
     "tp_graph_destroy": [_P],
     "tp_wall_clock_hz": [],
+    "tp_nn_input": [_G, _P, _I, _P, _SZ, _P, _P],
+    "tp_nn_forward": [_P, _P, _I, _P, _P, _P],
+    "tp_nn_backward": [_P, _P, _P, _P, _I, _F, _F, _F, _P, _P, _P],
+    "tp_nn_adam": [_P, _P, _P, _P, _I, _F, _F, _F, _F, _I, _P, _P],
+    "tp_nn_pack_bf16": [_P, _I, _P, _P],
+    "tp_fill": [_P, _SZ, _F, _P],
 }
 _RESTYPES = {"tp_last_error": ctypes.c_char_p, "tp_version": ctypes.c_char_p,
              "tp_wall_clock_hz": ctypes.c_double}

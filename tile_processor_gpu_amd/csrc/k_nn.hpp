@@ -1,0 +1,427 @@
+// k_nn.hpp - disparity-refinement CNN: input assembly, forward, backward,
+// Adam.  Not in the reference (its Java host took a parabolic/centroid
+// argmax of the correlation); this network replaces that sub-pixel step.
+//
+// Network (per tile, 15 x 15 pixels, zero padding):
+//   x  [7][225]   combined correlation + the 6 pair tiles mapped onto the
+//                 horizontal disparity axis (as tp_corr_combine maps them)
+//   h1 = relu(conv3x3(x; W1 [32][7][9], b1))
+//   h2 = relu(conv3x3(h1; W2 [32][32][9], b2))
+//   z  = conv3x3(h2; W3 [2][32][9], b3)           z0: logits, z1: confidence map
+//   p  = softmax(z0) over the 225 pixels
+//   disparity  = sum p (x - 7)                    (x = column, residual in px)
+//   confidence = sigmoid(sum p z1)
+// Parameters are BF16 (one flat array, layout NN_OFF_*), arithmetic FP32.
+//
+// Training loss per tile (mean over the tiles with a finite target):
+//   huber_delta(disparity - target) + conf_weight * BCE(confidence, |err| < 0.5)
+// with the BCE target held constant.
+//
+// One workgroup of 256 threads handles one tile at a time (thread = pixel,
+// threads 225..255 only help with loads and reductions).  Activations and
+// gradients of a tile live in LDS (<= 64 KB).  The backward pass keeps each
+// workgroup's weight-gradient sums in registers across its tiles and adds
+// them to the global gradient once at the end.
+#pragma once
+
+#include "tp_common.hpp"
+#include "k_combine.hpp"
+
+namespace tp {
+
+constexpr int NN_C = 32, NN_IN = 7, NN_P = 225;
+constexpr int NN_OFF_W1 = 0, NN_OFF_B1 = 2016, NN_OFF_W2 = 2048, NN_OFF_B2 = 11264,
+              NN_OFF_W3 = 11296, NN_OFF_B3 = 11872, NN_NPARAMS = 11874;
+constexpr int NN_ACTS = 2 * NN_C * NN_P + 2 * NN_P;  // h1, h2, z per tile
+
+__device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ uint16_t f2bf16(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// offset of neighbour r (3x3, row-major, r = 4 is the centre)
+__host__ __device__ constexpr int nbr_off(int r) { return (r / 3 - 1) * 15 + (r % 3 - 1); }
+
+// bit r set when neighbour r of pixel (y, x) lies inside the 15 x 15 tile
+__device__ __forceinline__ unsigned nbr_mask_yx(int y, int x) {
+    unsigned m = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        const int yy = y + r / 3 - 1, xx = x + r % 3 - 1;
+        if (yy >= 0 && yy < 15 && xx >= 0 && xx < 15) m |= 1u << r;
+    }
+    return m;
+}
+
+// value of channel plane `in` at neighbour r of pixel p (0 outside)
+__device__ __forceinline__ float nbr(const float *in, int p, unsigned m, int r) {
+    const int q = min(NN_P - 1, max(0, p + nbr_off(r)));
+    const float v = in[q];
+    return ((m >> r) & 1u) ? v : 0.f;
+}
+
+// acc[co] += sum_ci sum_r w[co][ci][r] in[ci][p + off_r]
+template <int CIN, int COUT>
+__device__ __forceinline__ void conv_px(const float *in, int p, unsigned m,
+                                        const uint16_t *__restrict__ w, float *acc) {
+    for (int ci = 0; ci < CIN; ci++) {
+        float v[9];
+#pragma unroll
+        for (int r = 0; r < 9; r++) v[r] = nbr(in + ci * NN_P, p, m, r);
+#pragma unroll
+        for (int co = 0; co < COUT; co++)
+#pragma unroll
+            for (int r = 0; r < 9; r++) acc[co] = fmaf(bf16f(w[(co * CIN + ci) * 9 + r]), v[r], acc[co]);
+    }
+}
+
+// block-wide sums of up to 3 values (256 threads)
+__device__ __forceinline__ void block_sum3(float &a, float &b, float &c, float *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+        c += __shfl_xor(c, o);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = a;
+        red[4 + w] = b;
+        red[8 + w] = c;
+    }
+    __syncthreads();
+    a = red[0] + red[1] + red[2] + red[3];
+    b = red[4] + red[5] + red[6] + red[7];
+    c = red[8] + red[9] + red[10] + red[11];
+}
+
+__device__ __forceinline__ float block_max(float a, float *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a = fmaxf(a, __shfl_xor(a, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[12 + (threadIdx.x >> 6)] = a;
+    __syncthreads();
+    return fmaxf(fmaxf(red[12], red[13]), fmaxf(red[14], red[15]));
+}
+
+// Softmax statistics of the head for pixel p: returns p_q (probability of
+// this pixel), the disparity and the confidence logit.
+__device__ __forceinline__ void head_stats(float z0, float z1, bool px, int xcol, float *red,
+                                           float &prob, float &disp, float &cl) {
+    const float mx = block_max(px ? z0 : -INFINITY, red);
+    const float e = px ? __expf(z0 - mx) : 0.f;
+    float s = e, sx = e * (float)(xcol - 7), sc = e * z1;
+    block_sum3(s, sx, sc, red);
+    const float inv = 1.0f / s;
+    prob = e * inv;
+    disp = sx * inv;
+    cl = sc * inv;
+}
+
+// ---------------------------------------------------------------- input
+// x[t][0] = sum of the mapped pair tiles, x[t][1 + p] = mapped tile of pair
+// p (zero when the pair is not set in the task or the tile is off-grid).
+// corrs: rows 6 t + p, 15 x 15 (radius 7).
+__global__ void k_nn_input(const tp_task *tasks, int num_tasks, int tiles_x, int tiles_y,
+                           const float *corrs, size_t corr_stride, float *x) {
+    const int t = blockIdx.x;
+    if (t >= num_tasks) return;
+    const tp_task tk = tasks[t];
+    const int tx = tk.txy & 0xfff, ty = tk.txy >> 16;
+    const bool ok = tx < tiles_x && ty >= 0 && ty < tiles_y;
+    for (int o = threadIdx.x; o < NN_P; o += blockDim.x) {
+        const int cy = o / 15 - 7, cx = o % 15 - 7;
+        float sum = 0.f;
+#pragma unroll
+        for (int p = 0; p < TP_NUM_PAIRS; p++) {
+            float v = 0.f;
+            int sy, sx;
+            if (ok && ((tk.task >> (TP_TASK_CORR_BITS + p)) & 1) && pair_src(p, 7, cy, cx, sy, sx))
+                v = corrs[(size_t)(6 * t + p) * corr_stride + (sy + 7) * 15 + sx + 7];
+            x[((size_t)t * NN_IN + 1 + p) * NN_P + o] = v;
+            sum += v;
+        }
+        x[(size_t)t * NN_IN * NN_P + o] = sum;
+    }
+}
+
+// -------------------------------------------------------------- forward
+// out[t] = (disparity, confidence); acts (optional) = h1, h2, z per tile.
+__global__ __launch_bounds__(256) void k_nn_forward(const uint16_t *__restrict__ prm,
+                                                    const float *__restrict__ x, int n, float *acts,
+                                                    float *out) {
+    __shared__ float A[NN_C * NN_P];
+    __shared__ float B[NN_C * NN_P];
+    __shared__ float red[16];
+    const int t = threadIdx.x;
+    const bool px = t < NN_P;
+    const int y = px ? t / 15 : 0, xc = px ? t % 15 : 0;
+    const unsigned m = px ? nbr_mask_yx(y, xc) : 0u;
+    for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
+        const float *xt = x + (size_t)tile * NN_IN * NN_P;
+        for (int i = t; i < NN_IN * NN_P; i += 256) A[i] = xt[i];
+        __syncthreads();
+        float *at = acts ? acts + (size_t)tile * NN_ACTS : nullptr;
+        float acc[NN_C];
+        if (px) {
+#pragma unroll
+            for (int co = 0; co < NN_C; co++) acc[co] = bf16f(prm[NN_OFF_B1 + co]);
+            conv_px<NN_IN, NN_C>(A, t, m, prm + NN_OFF_W1, acc);
+#pragma unroll
+            for (int co = 0; co < NN_C; co++) {
+                const float h = fmaxf(acc[co], 0.f);
+                B[co * NN_P + t] = h;
+                if (at) at[co * NN_P + t] = h;
+            }
+        }
+        __syncthreads();
+        if (px) {
+#pragma unroll
+            for (int co = 0; co < NN_C; co++) acc[co] = bf16f(prm[NN_OFF_B2 + co]);
+            conv_px<NN_C, NN_C>(B, t, m, prm + NN_OFF_W2, acc);
+            // A (the input) was last read before the previous barrier
+#pragma unroll
+            for (int co = 0; co < NN_C; co++) {
+                const float h = fmaxf(acc[co], 0.f);
+                A[co * NN_P + t] = h;
+                if (at) at[(NN_C + co) * NN_P + t] = h;
+            }
+        }
+        __syncthreads();
+        float z[2] = {bf16f(prm[NN_OFF_B3]), bf16f(prm[NN_OFF_B3 + 1])};
+        if (px) conv_px<NN_C, 2>(A, t, m, prm + NN_OFF_W3, z);
+        if (at && px) {
+            at[2 * NN_C * NN_P + t] = z[0];
+            at[2 * NN_C * NN_P + NN_P + t] = z[1];
+        }
+        float prob, disp, cl;
+        head_stats(z[0], z[1], px, xc, red, prob, disp, cl);
+        if (t == 0) {
+            out[2 * tile] = disp;
+            out[2 * tile + 1] = 1.0f / (1.0f + __expf(-cl));
+        }
+        __syncthreads();  // A and B are rewritten by the next tile
+    }
+}
+
+// ------------------------------------------------------------- backward
+struct NNBackArgs {
+    const uint16_t *prm;
+    const float *x;
+    const float *acts;
+    const float *target;  // residual disparity per tile; NaN: tile ignored
+    int n;
+    float conf_weight;
+    float huber_delta;
+    float scale;          // 1 / number of tiles with a finite target
+    float *grads;         // NN_NPARAMS, accumulated
+    float *loss;          // [1], accumulated
+};
+
+__global__ __launch_bounds__(256) void k_nn_backward(NNBackArgs a) {
+    __shared__ float P[NN_C * NN_P];  // h2, then dY2 (in place)
+    __shared__ float Q[NN_C * NN_P];  // h1, then dY1 (in place)
+    __shared__ float S[NN_IN * NN_P];  // dz (2 planes), then the input x
+    __shared__ float red[16];
+    const int t = threadIdx.x;
+    const bool px = t < NN_P;
+    const int y = px ? t / 15 : 0, xc = px ? t % 15 : 0;
+    const unsigned m = px ? nbr_mask_yx(y, xc) : 0u;
+    // weight-gradient sums of this workgroup
+    const int ci2 = t >> 3, cb2 = (t & 7) * 4;  // dW2: [cb2 .. cb2+3][ci2][9]
+    float g2[4][9];
+    float g3[3], g1[8];
+    float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f;  // db1 (threads 192..223), db2 (224..255), db3 (0, 1)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int r = 0; r < 9; r++) g2[j][r] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) g3[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) g1[k] = 0.f;
+    const uint16_t *W2 = a.prm + NN_OFF_W2, *W3 = a.prm + NN_OFF_W3;
+
+    for (int tile = blockIdx.x; tile < a.n; tile += gridDim.x) {
+        const float tgt = a.target[tile];
+        if (!(fabsf(tgt) < INFINITY)) continue;  // uniform: NaN/inf targets are ignored
+        const float *at = a.acts + (size_t)tile * NN_ACTS;
+        for (int i = t; i < NN_C * NN_P; i += 256) {
+            Q[i] = at[i];
+            P[i] = at[NN_C * NN_P + i];
+        }
+        // (0) head
+        const float z0 = px ? at[2 * NN_C * NN_P + t] : 0.f;
+        const float z1 = px ? at[2 * NN_C * NN_P + NN_P + t] : 0.f;
+        float prob, disp, cl;
+        head_stats(z0, z1, px, xc, red, prob, disp, cl);  // contains barriers
+        const float e = disp - tgt;
+        const float hd = fminf(fmaxf(e, -a.huber_delta), a.huber_delta);
+        const float sg = 1.0f / (1.0f + __expf(-cl));
+        const float lab = fabsf(e) < 0.5f ? 1.f : 0.f;
+        const float gc = a.conf_weight * (sg - lab);
+        if (px) {
+            S[t] = a.scale * prob * (hd * (float)(xc - 7) - hd * disp + gc * (z1 - cl));
+            S[NN_P + t] = a.scale * gc * prob;
+        }
+        if (t == 0) {
+            const float ae = fabsf(e);
+            const float hub = ae <= a.huber_delta ? 0.5f * e * e : a.huber_delta * (ae - 0.5f * a.huber_delta);
+            const float bce = -(lab * logf(fmaxf(sg, 1e-12f)) + (1.f - lab) * logf(fmaxf(1.f - sg, 1e-12f)));
+            atomicAdd(a.loss, a.scale * (hub + a.conf_weight * bce));
+        }
+        __syncthreads();
+        // (1) dW3 [2][32][9], db3: 576 + 2 sums over pixels of dz * h2 neighbours
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int i = t + 256 * k;
+            if (i < 576) {
+                const int co = i / 288, ci = (i / 9) % 32, r = i % 9;
+                float s = 0.f;
+                for (int yy = 0; yy < 15; yy++)
+                    for (int xx = 0; xx < 15; xx++) {
+                        const int p = yy * 15 + xx;
+                        const int sy = yy + r / 3 - 1, sx = xx + r % 3 - 1;
+                        if (sy >= 0 && sy < 15 && sx >= 0 && sx < 15)
+                            s = fmaf(S[co * NN_P + p], P[ci * NN_P + sy * 15 + sx], s);
+                    }
+                g3[k] += s;
+            }
+        }
+        if (t < 2) {  // db3
+            float s = 0.f;
+            for (int p = 0; p < NN_P; p++) s += S[t * NN_P + p];
+            gb3 += s;
+        }
+        __syncthreads();
+        // (2) dY2 = (W3^T * dz) . [h2 > 0], in place over h2 (own pixel only)
+        if (px) {
+            float acc[NN_C];
+#pragma unroll
+            for (int ci = 0; ci < NN_C; ci++) acc[ci] = 0.f;
+#pragma unroll
+            for (int co = 0; co < 2; co++) {
+                float v[9];
+#pragma unroll
+                for (int r = 0; r < 9; r++) v[r] = nbr(S + co * NN_P, t, m, 8 - r);  // dz[q - off_r]
+#pragma unroll
+                for (int ci = 0; ci < NN_C; ci++)
+#pragma unroll
+                    for (int r = 0; r < 9; r++) acc[ci] = fmaf(bf16f(W3[(co * NN_C + ci) * 9 + r]), v[r], acc[ci]);
+            }
+#pragma unroll
+            for (int ci = 0; ci < NN_C; ci++)
+                P[ci * NN_P + t] = P[ci * NN_P + t] > 0.f ? acc[ci] : 0.f;
+        }
+        __syncthreads();
+        // input x into S (dz no longer needed)
+        for (int i = t; i < NN_IN * NN_P; i += 256) S[i] = a.x[(size_t)tile * NN_IN * NN_P + i];
+        // (3) dW2 [32][32][9]: thread (ci2, cb2..cb2+3); db2: threads 224..255
+        for (int yy = 0; yy < 15; yy++)
+            for (int xx = 0; xx < 15; xx++) {
+                const int p = yy * 15 + xx;
+                const unsigned mm = nbr_mask_yx(yy, xx);
+                float h[9];
+#pragma unroll
+                for (int r = 0; r < 9; r++) h[r] = nbr(Q + ci2 * NN_P, p, mm, r);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float d = P[(cb2 + j) * NN_P + p];
+#pragma unroll
+                    for (int r = 0; r < 9; r++) g2[j][r] = fmaf(d, h[r], g2[j][r]);
+                }
+            }
+        if (t >= 224) {  // db2 channel t - 224
+            float s = 0.f;
+            for (int p = 0; p < NN_P; p++) s += P[(t - 224) * NN_P + p];
+            gb2 += s;
+        }
+        __syncthreads();
+        // (4) dY1 = (W2^T * dY2) . [h1 > 0], in place over h1 (own pixel only)
+        if (px) {
+            float acc[NN_C];
+#pragma unroll
+            for (int ci = 0; ci < NN_C; ci++) acc[ci] = 0.f;
+            for (int co = 0; co < NN_C; co++) {
+                float v[9];
+#pragma unroll
+                for (int r = 0; r < 9; r++) v[r] = nbr(P + co * NN_P, t, m, 8 - r);
+#pragma unroll
+                for (int ci = 0; ci < NN_C; ci++)
+#pragma unroll
+                    for (int r = 0; r < 9; r++) acc[ci] = fmaf(bf16f(W2[(co * NN_C + ci) * 9 + r]), v[r], acc[ci]);
+            }
+#pragma unroll
+            for (int ci = 0; ci < NN_C; ci++)
+                Q[ci * NN_P + t] = Q[ci * NN_P + t] > 0.f ? acc[ci] : 0.f;
+        }
+        __syncthreads();
+        // (5) dW1 [32][7][9] (8 per thread) and db1 (threads 192..223)
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int i = t + 256 * k;
+            if (i < 2016) {
+                const int co = i / 63, ci = (i / 9) % 7, r = i % 9;
+                float s = 0.f;
+                for (int yy = 0; yy < 15; yy++)
+                    for (int xx = 0; xx < 15; xx++) {
+                        const int sy = yy + r / 3 - 1, sx = xx + r % 3 - 1;
+                        if (sy >= 0 && sy < 15 && sx >= 0 && sx < 15)
+                            s = fmaf(Q[co * NN_P + yy * 15 + xx], S[ci * NN_P + sy * 15 + sx], s);
+                    }
+                g1[k] += s;
+            }
+        }
+        if (t >= 192 && t < 224) {
+            float s = 0.f;
+            for (int p = 0; p < NN_P; p++) s += Q[(t - 192) * NN_P + p];
+            gb1 += s;
+        }
+        __syncthreads();  // P, Q, S are rewritten by the next tile
+    }
+    // add this workgroup's sums to the global gradient
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int r = 0; r < 9; r++) atomicAdd(&a.grads[NN_OFF_W2 + ((cb2 + j) * NN_C + ci2) * 9 + r], g2[j][r]);
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (t + 256 * k < 576) atomicAdd(&a.grads[NN_OFF_W3 + t + 256 * k], g3[k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (t + 256 * k < 2016) atomicAdd(&a.grads[NN_OFF_W1 + t + 256 * k], g1[k]);
+    if (t >= 192 && t < 224) atomicAdd(&a.grads[NN_OFF_B1 + t - 192], gb1);
+    if (t >= 224) atomicAdd(&a.grads[NN_OFF_B2 + t - 224], gb2);
+    if (t < 2) atomicAdd(&a.grads[NN_OFF_B3 + t], gb3);
+}
+
+// ---------------------------------------------------------------- Adam
+__global__ void k_nn_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1,
+                          float b2, float eps, float c1, float c2, uint16_t *wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = fmaf(b1, m[i], (1.f - b1) * gi);
+    const float vi = fmaf(b2, v[i], (1.f - b2) * gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    wb[i] = f2bf16(wi);
+}
+
+__global__ void k_nn_pack(const float *w, int n, uint16_t *wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = f2bf16(w[i]);
+}
+
+__global__ void k_fill(float *p, size_t n, float v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace tp

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "tp_common.hpp"
@@ -16,6 +17,7 @@
 #include "k_texture.hpp"
 #include "k_combine.hpp"
 #include "k_fused.hpp"
+#include "k_nn.hpp"
 
 namespace {
 
@@ -51,6 +53,8 @@ int check_geometry(const tp_geometry *g) {
 inline hipStream_t as_stream(tp_stream_t s) { return (hipStream_t)s; }
 
 static_assert(sizeof(tp_frame) == sizeof(tp::FrameDesc), "tp_frame layout");
+static_assert(TP_NN_PARAMS == tp::NN_NPARAMS && TP_NN_ACTS == tp::NN_ACTS &&
+              TP_NN_INPUT == tp::NN_IN * tp::NN_P, "CNN sizes");
 static_assert(offsetof(tp_frame, item_start) == offsetof(tp::FrameDesc, item_start), "tp_frame layout");
 
 struct FusedCfg {
@@ -605,6 +609,97 @@ double tp_wall_clock_hz(void) {
     if (hipGetDevice(&dev) != hipSuccess) return 0.0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0.0;
     return 1e3 * khz;
+}
+
+int tp_nn_input(const tp_geometry *geom, const tp_task *gpu_tasks, int num_tasks,
+                const float *gpu_corrs, size_t corr_stride, float *gpu_x, tp_stream_t stream) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (!gpu_tasks || !gpu_corrs || !gpu_x) return fail(TP_EINVAL, "tp_nn_input: NULL pointer argument%s", "");
+    if (num_tasks < 0) return fail(TP_EINVAL, "tp_nn_input: num_tasks < 0%s", "");
+    if (corr_stride < 225) return fail(TP_EINVAL, "tp_nn_input: corr_stride < 225 (radius 7 rows)%s", "");
+    g_err.clear();
+    if (!num_tasks) return TP_OK;
+    hipLaunchKernelGGL(tp::k_nn_input, dim3(num_tasks), dim3(256), 0, as_stream(stream), gpu_tasks,
+                       num_tasks, geom->img_width / 8, geom->img_height / 8, gpu_corrs, corr_stride, gpu_x);
+    return check_launch("nn_input");
+}
+
+namespace {
+int nn_grid(int n) {
+    const int cus = num_cus();
+    const int g = (cus > 0 ? cus : 256) * 4;
+    return std::max(1, std::min(n, g));
+}
+}  // namespace
+
+int tp_nn_forward(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_acts,
+                  float *gpu_out, tp_stream_t stream) {
+    if (!gpu_params_bf16 || !gpu_x || !gpu_out) return fail(TP_EINVAL, "tp_nn_forward: NULL pointer argument%s", "");
+    if (n < 0) return fail(TP_EINVAL, "tp_nn_forward: n < 0%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    hipLaunchKernelGGL(tp::k_nn_forward, dim3(nn_grid(n)), dim3(256), 0, as_stream(stream),
+                       gpu_params_bf16, gpu_x, n, gpu_acts, gpu_out);
+    return check_launch("nn_forward");
+}
+
+int tp_nn_backward(const uint16_t *gpu_params_bf16, const float *gpu_x, const float *gpu_acts,
+                   const float *gpu_target, int n, float conf_weight, float huber_delta,
+                   float loss_scale, float *gpu_grads, float *gpu_loss, tp_stream_t stream) {
+    if (!gpu_params_bf16 || !gpu_x || !gpu_acts || !gpu_target || !gpu_grads || !gpu_loss)
+        return fail(TP_EINVAL, "tp_nn_backward: NULL pointer argument%s", "");
+    if (n < 0) return fail(TP_EINVAL, "tp_nn_backward: n < 0%s", "");
+    if (!(huber_delta > 0.f)) return fail(TP_EINVAL, "tp_nn_backward: huber_delta must be positive%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    tp::NNBackArgs a;
+    a.prm = gpu_params_bf16;
+    a.x = gpu_x;
+    a.acts = gpu_acts;
+    a.target = gpu_target;
+    a.n = n;
+    a.conf_weight = conf_weight;
+    a.huber_delta = huber_delta;
+    a.scale = loss_scale;
+    a.grads = gpu_grads;
+    a.loss = gpu_loss;
+    hipLaunchKernelGGL(tp::k_nn_backward, dim3(nn_grid(n)), dim3(256), 0, as_stream(stream), a);
+    return check_launch("nn_backward");
+}
+
+int tp_nn_adam(float *gpu_params, float *gpu_m, float *gpu_v, const float *gpu_grads, int n,
+               float lr, float beta1, float beta2, float eps, int step, uint16_t *gpu_params_bf16,
+               tp_stream_t stream) {
+    if (!gpu_params || !gpu_m || !gpu_v || !gpu_grads || !gpu_params_bf16)
+        return fail(TP_EINVAL, "tp_nn_adam: NULL pointer argument%s", "");
+    if (n < 0 || step < 1) return fail(TP_EINVAL, "tp_nn_adam: n < 0 or step < 1%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, step)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, step)));
+    hipLaunchKernelGGL(tp::k_nn_adam, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), gpu_params,
+                       gpu_m, gpu_v, gpu_grads, n, lr, beta1, beta2, eps, c1, c2, gpu_params_bf16);
+    return check_launch("nn_adam");
+}
+
+int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t stream) {
+    if (!gpu_src || !gpu_dst) return fail(TP_EINVAL, "tp_nn_pack_bf16: NULL pointer argument%s", "");
+    if (n < 0) return fail(TP_EINVAL, "tp_nn_pack_bf16: n < 0%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    hipLaunchKernelGGL(tp::k_nn_pack, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), gpu_src, n,
+                       gpu_dst);
+    return check_launch("nn_pack_bf16");
+}
+
+int tp_fill(float *gpu_dst, size_t n, float value, tp_stream_t stream) {
+    if (!gpu_dst) return fail(TP_EINVAL, "tp_fill: NULL pointer argument%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    hipLaunchKernelGGL(tp::k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream),
+                       gpu_dst, n, value);
+    return check_launch("fill");
 }
 
 }  // extern "C"
