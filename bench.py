@@ -202,6 +202,67 @@ def measure_stream(nframes, reps, seed=4):
             "config": cfg, "tuned_over": len(report), "graph": True}
 
 
+def measure_stages(cfg, fb, idx, reps):
+    """The rest of the full-frame pipeline, timed separately (not part of
+    `value`): six-pair combination to 15 x 15 tiles, texture output
+    (generate_RBGA: list, texture tiles, RGBA overlap-add) and the
+    disparity-refinement network (input assembly + forward)."""
+    import ctypes
+    import torch
+    from tile_processor_gpu_amd import _lib, nn, tp
+    from tile_processor_gpu_amd.geometry import TASK_DTYPE
+    g = cfg["geom"]
+    out = {}
+    dev = torch.device("cuda:0")
+    # staged CLT + correlation list in fb
+    fb.convert()
+    fb.correlate(3, SCALES, FAT_ZERO)
+    comb = torch.zeros((g.num_tiles, 225), device=dev)
+    ms = median_ms(lambda: tp.corr_combine(g, fb.num_corr, fb.corr_idx, fb.corrs, fb.corr_stride,
+                                           RADIUS, 0x3F, comb, 225), reps)
+    out["combine_ms"] = round(ms, 4)
+    disp = torch.zeros((g.num_tiles, 2), device=dev)
+    out["centroid_ms"] = round(median_ms(lambda: tp.corr_disparity(comb, 225, g.num_tiles, RADIUS, 1,
+                                                                    disp), reps), 4)
+    # texture output
+    t = np.ascontiguousarray(cfg["tasks"])
+    tasks0 = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
+    tasks = tasks0.clone()
+    ind = torch.zeros(g.tilesx * g.tilesya, dtype=torch.int32, device=dev)
+    num = torch.zeros(8, dtype=torch.int32, device=dev)
+    woi = torch.zeros(4, dtype=torch.int32, device=dev)
+    stride = (g.tilesx + 1) * 8
+    rbga = torch.zeros((4 * (g.tilesy + 1) * 8, stride), device=dev)
+    ports = torch.tensor([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]], device=dev)
+    kw = tp.TEXTURE_DEFAULTS
+
+    def texture():
+        tasks.copy_(tasks0)
+        tp.generate_RBGA(g, tasks, len(t), ind, num, woi, g.tilesx, g.tilesy, fb.p_clt, ports,
+                         3, 0, kw["min_shot"], kw["scale_shot"], kw["diff_sigma"], kw["diff_threshold"],
+                         kw["min_agree"], kw["weight0"], kw["weight1"], kw["weight2"],
+                         kw["dust_remove"], 0, stride, rbga)
+    out["texture_rgba_ms"] = round(median_ms(texture, max(3, reps // 2)), 4)
+    # network on the full frame
+    if os.path.exists(nn.DEFAULT_WEIGHTS):
+        model = nn.DisparityCNN.load(nn.DEFAULT_WEIGHTS)
+        fb.fused(3, SCALES, FAT_ZERO)
+        x = torch.empty((g.num_tiles, 7, 225), device=dev)
+
+        def nn_frame():
+            _lib.call("tp_nn_input", ctypes.byref(g.c()), ctypes.c_void_p(fb.tasks.data_ptr()),
+                      g.num_tiles, ctypes.c_void_p(fb.corrs.data_ptr()), fb.corr_stride,
+                      ctypes.c_void_p(x.data_ptr()), tp._stream(None))
+            model.forward(x, g.num_tiles)
+        out["nn_frame_ms"] = round(median_ms(nn_frame, max(3, reps // 4), warmup=1), 4)
+        del x
+        res = nn.evaluate(model, nn.TileSource(4, seed=777), batches=2)
+        out["nn_accuracy"] = {"data": "rendered piecewise-planar scenes, 4 x 256 x 256, "
+                                      "residual U[-2, 2] px, noise 0.5-5%, unseen seeds",
+                              **res}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,7 +275,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the plain staged path (no fused / stream measurements)")
     ap.add_argument("--stream-frames", type=int, default=64)
-    ap.add_argument("--mode", choices=["frame", "copy"], default="frame",
+    ap.add_argument("--mode", choices=["frame", "copy", "fused"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
     args = ap.parse_args()
@@ -237,6 +298,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
                           "copies": max(1, args.steps)}), flush=True)
+        return
+
+    if args.mode == "fused":  # the fused kernel alone (default launch config), for profiling
+        cfg = synth.config_full(seed=args.seed)
+        g = cfg["geom"]
+        idx = corr_indices(g, cfg["tasks"])
+        fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
+        fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+        ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO), max(1, args.steps), args.warmup)
+        print(json.dumps({"mode": "fused", "ms": round(ms, 4), "steps": args.steps}), flush=True)
         return
 
     t0 = time.time()
@@ -309,6 +380,8 @@ def main():
         log(f"fused: {extra['fused']}")
         extra["stream"] = measure_stream(args.stream_frames, max(5, K // 2))
         log(f"stream: {extra['stream']}")
+        extra["pipeline_stages"] = measure_stages(cfg, fb, idx, max(5, K // 2))
+        log(f"stages: {extra['pipeline_stages']}")
 
     cpu = None
     if args.cpu_rows > 0:
