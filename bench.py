@@ -77,23 +77,34 @@ def err_vs_fp64(cfg, fb, idx, sample_tiles):
     tasks = cfg["tasks"][sample_tiles]
     clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], tasks)
     torch.cuda.synchronize()
-    worst_clt = 0.0
+    worst_clt, rms_clt = 0.0, 0.0
     for cam in range(4):
         got = fb.clt[cam].view(-1, 3, 256)[torch.as_tensor(sample_tiles, device=fb.device)]
         got = got.cpu().numpy().astype(np.float64)
         ref = clt64[cam].reshape(-1, 3, 256)[sample_tiles]
         for color in range(3):
-            e = np.abs(got[:, color] - ref[:, color]).max() / np.abs(ref[:, color]).max()
-            worst_clt = max(worst_clt, float(e))
+            den = np.abs(ref[:, color]).max()
+            d = got[:, color] - ref[:, color]
+            worst_clt = max(worst_clt, float(np.abs(d).max() / den))
+            rms_clt = max(rms_clt, float(np.sqrt(np.mean(d * d)) / den))
     sel = np.nonzero(np.isin(idx >> 8, sample_tiles))[0]
     ref = orc.correlate2D(g, clt64, 3, SCALES, FAT_ZERO, idx[sel], RADIUS)
     got = fb.corrs[torch.as_tensor(sel, device=fb.device)].cpu().numpy().astype(np.float64)
-    e_corr = float(np.abs(got[:, :ref.shape[1]] - ref).max() / np.abs(ref).max())
     n = 2 * RADIUS + 1
-    pk_got = got[:, :n * n].argmax(1)
+    got = got[:, :n * n]
+    den = np.abs(ref).max()
+    e_corr = float(np.abs(got - ref).max() / den)
+    rms_corr = float(np.sqrt(np.mean((got - ref) ** 2)) / den)
+    pk_got = got.argmax(1)
     pk_ref = ref.argmax(1)
-    return {"clt": worst_clt, "corr": e_corr,
-            "argmax_mismatch": int((pk_got != pk_ref).sum()), "sample_tiles": int(len(sample_tiles))}
+    # sub-pixel disparity of the combined tiles (centroid), GPU vs FP64
+    out = {"clt": worst_clt, "clt_rms": rms_clt, "corr": e_corr, "corr_rms": rms_corr,
+           "argmax_mismatch": int((pk_got != pk_ref).sum()), "sample_tiles": int(len(sample_tiles))}
+    if len(sel) % 6 == 0 and len(sel):
+        d_got = orc.corr_disparity(orc.corr_combine(got, RADIUS), RADIUS, mode=1)[:, 0]
+        d_ref = orc.corr_disparity(orc.corr_combine(ref, RADIUS), RADIUS, mode=1)[:, 0]
+        out["disparity_max_abs_px"] = float(np.abs(d_got - d_ref).max())
+    return out
 
 
 def cpu_baseline(cfg, idx, rows, reps=3):

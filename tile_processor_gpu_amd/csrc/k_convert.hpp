@@ -26,6 +26,7 @@
 #pragma once
 
 #include "tp_common.hpp"
+#include "mfma_dct.hpp"
 
 namespace tp {
 
@@ -125,13 +126,35 @@ __device__ __forceinline__ LaneFold lane_fold(int li) {
     return f;
 }
 
+// Matrix-core constants of the split-precision forward DCT (DCT = 1: FP16,
+// 2: BF16 hi + lo), per lane; DCT = 0 (FP32 butterflies) needs none.
+template <int DCT>
+struct CvtMfma {
+    MfmaConst<DCT> c4, s4;
+};
+template <>
+struct CvtMfma<0> {};
+
+template <int DCT>
+__device__ __forceinline__ CvtMfma<DCT> cvt_mfma(int lane) {
+    CvtMfma<DCT> t;
+    if constexpr (DCT != 0) {
+        t.c4 = mfma_const<DCT, 0>(lane);
+        t.s4 = mfma_const<DCT, 1>(lane);
+    }
+    return t;
+}
+
 // One unit (tile task_num, camera cam, color COLOR) up to the optional LPF:
-// on return lane li holds row l = li of the 4 quadrants in q0..q3.
-template <int COLOR>
+// on return lane li holds row l = li of the 4 quadrants in q0..q3.  DCT
+// selects the 8-point DCT-IV/DST-IV stages: FP32 butterflies (0) or
+// split-precision matrix-core contractions across the wavefront's 8 units
+// (1: FP16 hi + lo, 2: BF16 hi + lo).
+template <int COLOR, int DCT = 0>
 __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
                                              float *lds_s, int li, int cam, int task_num,
                                              float (&q0)[8], float (&q1)[8], float (&q2)[8],
-                                             float (&q3)[8]) {
+                                             float (&q3)[8], const CvtMfma<DCT> &mt = {}) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
     const tp_task *tk = a.tasks + task_num;
@@ -223,17 +246,23 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
         }
     }
 
-    // (5) vertical DCT-IV (column li)
-    dct4_8(f);
-    if (color1) dct4_8(fs);
-
-    // (6) transpose through LDS: lane li writes column li as row li of the
-    // staging block, then reads its row li (element k from lane k).
-    *(float4 *)(lds_c + li * 8) = make_float4(f[0], f[1], f[2], f[3]);
-    *(float4 *)(lds_c + li * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
-    if (color1) {
-        *(float4 *)(lds_s + li * 8) = make_float4(fs[0], fs[1], fs[2], fs[3]);
-        *(float4 *)(lds_s + li * 8 + 4) = make_float4(fs[4], fs[5], fs[6], fs[7]);
+    // (5) vertical DCT-IV (column li), (6) transpose through LDS: column li
+    // goes to row li of the staging block, then lane li reads its row li
+    // (element k from lane k).
+    const int lane = __lane_id();
+    float *stage_c = lds_c - (lane >> 3) * 72, *stage_s = lds_s - (lane >> 3) * 72;
+    if constexpr (DCT == 0) {
+        dct4_8(f);
+        if (color1) dct4_8(fs);
+        *(float4 *)(lds_c + li * 8) = make_float4(f[0], f[1], f[2], f[3]);
+        *(float4 *)(lds_c + li * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
+        if (color1) {
+            *(float4 *)(lds_s + li * 8) = make_float4(fs[0], fs[1], fs[2], fs[3]);
+            *(float4 *)(lds_s + li * 8 + 4) = make_float4(fs[4], fs[5], fs[6], fs[7]);
+        }
+    } else {
+        mfma_dct8_to_lds<DCT>(f, mt.c4, stage_c, lane);
+        if (color1) mfma_dct8_to_lds<DCT>(fs, mt.c4, stage_s, lane);
     }
     wave_lds_sync();
     float cc[8], sc[8];
@@ -246,8 +275,21 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     wave_lds_sync();
 
     // (7) horizontal DCT-IV (and DST-IV for G)
-    dct4_8(cc);
-    if (color1) dst4_8(sc);
+    if constexpr (DCT == 0) {
+        dct4_8(cc);
+        if (color1) dst4_8(sc);
+    } else {
+        mfma_dct8_to_lds<DCT>(cc, mt.c4, stage_c, lane);
+        if (color1) mfma_dct8_to_lds<DCT>(sc, mt.s4, stage_s, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < 8; k++) cc[k] = lds_c[li * 8 + k];
+        if (color1) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) sc[k] = lds_s[li * 8 + k];
+        }
+        wave_lds_sync();
+    }
 
     // (8) replicate quadrants, :3457-3497.  Lane li is row l = li.
     const float nd = (float)(1 - ((((itlx & 1) ^ (itly & 1) ^ (1 ^ color1))) << 1));

@@ -57,7 +57,7 @@ struct FusedArgs {
     int radius;
 };
 
-template <int GT, bool QUEUE, int COLORS>
+template <int GT, bool QUEUE, int COLORS, int DCT = 0>
 __global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
     static_assert(GT == 2 || GT == 4, "tiles per group");
     constexpr int NSLOT = 6 * GT;
@@ -72,6 +72,7 @@ __global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
     float *lds_c = &cvt_lds[wave][0][unit][0];
     float *lds_s = &cvt_lds[wave][1][unit][0];
     const LaneFold lf = lane_fold(li);
+    const CvtMfma<DCT> mt = cvt_mfma<DCT>(lane);
     int src_off[4];
     corr_out_offsets(lane, a.radius, src_off);
     const float fz2 = a.fat_zero * a.fat_zero;
@@ -125,13 +126,13 @@ __global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
                 float q0[8], q1[8], q2[8], q3[8];
                 int color;
                 if (wave == 0) {
-                    convert_unit<0>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    convert_unit<0, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
                     color = 0;
                 } else if (wave == 1) {
-                    convert_unit<1>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    convert_unit<1, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
                     color = 1;
                 } else {
-                    convert_unit<2>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+                    convert_unit<2, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
                     color = 2;
                 }
                 float *d = clt_lds + ((tl * 4 + cam) * 3 + color) * 256 + li * 8;

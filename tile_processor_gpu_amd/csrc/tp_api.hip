@@ -68,7 +68,7 @@ int fused_config(const char *fn, const tp_fused_config *c, FusedCfg &o) {
     if (o.tpi < o.gt || o.tpi % o.gt || o.tpi > 4096)
         return fail(TP_EINVAL, "%s: tiles_per_item must be a multiple of tiles_per_group, <= 4096", fn);
     if (o.bpc < 1 || o.bpc > 8) return fail(TP_EINVAL, "%s: blocks_per_cu must be 1..8", fn);
-    if (o.dct != 0) return fail(TP_EINVAL, "%s: dct must be 0", fn);
+    if (o.dct < 0 || o.dct > 2) return fail(TP_EINVAL, "%s: dct must be 0, 1 or 2", fn);
     return TP_OK;
 }
 
@@ -102,16 +102,23 @@ tp::FusedArgs fused_args(const tp_geometry *geom, size_t dstride, int lpf_mask, 
     return a;
 }
 
-template <bool Q>
-void launch_fused(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
+template <bool Q, int DCT>
+void launch_fused_d(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
     const dim3 gd(grid), bd(192);
     if (c.gt == 2) {
-        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<2, Q, 3>), gd, bd, 0, st, a);
-        else hipLaunchKernelGGL((tp::k_fused<2, Q, 1>), gd, bd, 0, st, a);
+        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<2, Q, 3, DCT>), gd, bd, 0, st, a);
+        else hipLaunchKernelGGL((tp::k_fused<2, Q, 1, DCT>), gd, bd, 0, st, a);
     } else {
-        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<4, Q, 3>), gd, bd, 0, st, a);
-        else hipLaunchKernelGGL((tp::k_fused<4, Q, 1>), gd, bd, 0, st, a);
+        if (colors == 3) hipLaunchKernelGGL((tp::k_fused<4, Q, 3, DCT>), gd, bd, 0, st, a);
+        else hipLaunchKernelGGL((tp::k_fused<4, Q, 1, DCT>), gd, bd, 0, st, a);
     }
+}
+
+template <bool Q>
+void launch_fused(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
+    if (c.dct == 1) launch_fused_d<Q, 1>(a, c, colors, grid, st);
+    else if (c.dct == 2) launch_fused_d<Q, 2>(a, c, colors, grid, st);
+    else launch_fused_d<Q, 0>(a, c, colors, grid, st);
 }
 
 int num_cus() {
