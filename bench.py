@@ -148,25 +148,44 @@ def median_ms(fn, reps, warmup=3):
     return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
-def measure_fused(fb, reps, staged_ms, frame_bytes_fused):
-    """Fused forward CLT + correlation of the same frame (CLT kept on chip),
-    with the autotuned launch configuration; correlations checked against
-    the staged path's."""
+METHODS = (("fp32", 0), ("mfma_fp16x2", 1), ("mfma_bf16x2", 2))
+
+
+def measure_fused(cfg, fb, idx, reps, staged_ms, frame_bytes_fused, sample):
+    """Fused forward CLT + correlation of the same frame (CLT kept on chip)
+    for each forward-DCT method (FP32 butterflies; matrix-core FP16 and BF16
+    hi + lo), autotuned; also as one persistent launch replayed from a
+    captured graph; errors against the FP64 oracle on the standard frame and
+    on a high-dynamic-range frame (pixels log-uniform over 1e-3 .. 1e3)."""
     import torch
-    from tile_processor_gpu_amd import autotune
+    from tile_processor_gpu_amd import autotune, synth, tp
     ref = fb.corrs[:fb.num_corr].clone()
-    report = []
-    cfg = autotune.tune_fused(fb, 3, SCALES, FAT_ZERO, cache_path=None, report=report)
-    ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, config=cfg), reps)
-    ms_clt = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, store_clt=True, config=cfg), reps)
-    dev = float((fb.corrs[:fb.num_corr] - ref).abs().max() / ref.abs().max())
-    torch.cuda.synchronize()
-    return {"frames_per_s": round(1e3 / ms, 2), "ms": round(ms, 4),
-            "speedup_vs_plain": round(staged_ms / ms, 3),
-            "ms_with_clt_store": round(ms_clt, 4),
-            "hbm_GBps_algorithmic": round(frame_bytes_fused / (ms * 1e-3) / 1e9, 1),
-            "config": cfg, "max_rel_dev_vs_plain": dev,
-            "tuned_over": len(report)}
+    hdr = dict(cfg, images=synth.make_images(cfg["geom"], np.random.default_rng(99), hdr=True))
+    out = {}
+    for name, dct in METHODS:
+        report = []
+        c = autotune.tune_fused(fb, 3, SCALES, FAT_ZERO, cache_path=None, report=report, dct=dct)
+        ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, config=c), reps)
+        run = tp.StreamRun([fb], 3, SCALES, FAT_ZERO, config=c)
+        run.capture()
+        ms_graph = median_ms(run.replay, reps)
+        run.release()
+        dev = float((fb.corrs[:fb.num_corr] - ref).abs().max() / ref.abs().max())
+        fb.fused(3, SCALES, FAT_ZERO, store_clt=True, config=c)
+        err = err_vs_fp64(cfg, fb, idx, sample)
+        fb.upload(hdr["images"])
+        fb.fused(3, SCALES, FAT_ZERO, store_clt=True, config=c)
+        err_hdr = err_vs_fp64(hdr, fb, idx, sample)
+        fb.upload(cfg["images"])
+        torch.cuda.synchronize()
+        best = min(ms, ms_graph)
+        out[name] = {"frames_per_s": round(1e3 / best, 2), "ms": round(ms, 4),
+                     "ms_persistent_graph": round(ms_graph, 4),
+                     "speedup_vs_plain": round(staged_ms / best, 3),
+                     "hbm_GBps_algorithmic": round(frame_bytes_fused / (best * 1e-3) / 1e9, 1),
+                     "config": c, "tuned_over": len(report), "max_rel_dev_vs_plain": dev,
+                     "err_vs_fp64": err, "err_vs_fp64_hdr": err_hdr}
+    return out
 
 
 def measure_stream(nframes, reps, seed=4):
@@ -387,7 +406,11 @@ def main():
     extra = {}
     if not args.no_extra:
         fused_bytes = (conv_b - 4 * g.num_tiles * 3 * 1024) + (corr_b - 4 * g.num_tiles * 3 * 1024)
-        extra["fused"] = measure_fused(fb, max(5, K // 2), 1e3 * total_s / K, fused_bytes)
+        rng = np.random.default_rng(11)
+        sample = np.sort(rng.choice(g.num_tiles, size=min(max(args.err_tiles, 64), g.num_tiles),
+                                    replace=False))
+        extra["fused"] = measure_fused(cfg, fb, idx, max(5, K // 2), 1e3 * total_s / K, fused_bytes,
+                                       sample)
         log(f"fused: {extra['fused']}")
         extra["stream"] = measure_stream(args.stream_frames, max(5, K // 2))
         log(f"stream: {extra['stream']}")

@@ -101,14 +101,16 @@ def tune_stream(frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, reps=
 
 
 def tune_fused(fb, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, store_clt=False, reps=5,
-               warmup=2, cache_path=DEFAULT_CACHE, retune=False, report=None):
-    """Fastest tp_fused_config for the single-frame fused launch on fb."""
-    key = _key(f"fused{'+clt' if store_clt else ''}", [fb])
+               warmup=2, cache_path=DEFAULT_CACHE, retune=False, report=None, dct=0):
+    """Fastest tp_fused_config (for the given forward-DCT method) for the
+    single-frame fused launch on fb."""
+    key = _key(f"fused{'+clt' if store_clt else ''}|dct={dct}", [fb])
     cache = _load(cache_path) if cache_path else {}
     if not retune and key in cache:
         return dict(cache[key]["config"])
     best, best_ms = None, float("inf")
     for c in candidates(stream=False):
+        c = dict(c, dct=dct)
         ms = _median_ms(lambda: fb.fused(colors, scales, fat_zero, store_clt=store_clt, config=c),
                         reps, warmup)
         if report is not None:

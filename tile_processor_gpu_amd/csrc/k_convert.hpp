@@ -146,7 +146,9 @@ __device__ __forceinline__ CvtMfma<DCT> cvt_mfma(int lane) {
 }
 
 // One unit (tile task_num, camera cam, color COLOR) up to the optional LPF:
-// on return lane li holds row l = li of the 4 quadrants in q0..q3.  DCT
+// on return lane li holds row l = li of the 4 quadrants in q0..q3.  lds_c,
+// lds_s: this unit's two 8 x 8 LDS staging blocks (the 8 units of a
+// wavefront are ustride floats apart).  DCT
 // selects the 8-point DCT-IV/DST-IV stages: FP32 butterflies (0) or
 // split-precision matrix-core contractions across the wavefront's 8 units
 // (1: FP16 hi + lo, 2: BF16 hi + lo).
@@ -154,7 +156,8 @@ template <int COLOR, int DCT = 0>
 __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
                                              float *lds_s, int li, int cam, int task_num,
                                              float (&q0)[8], float (&q1)[8], float (&q2)[8],
-                                             float (&q3)[8], const CvtMfma<DCT> &mt = {}) {
+                                             float (&q3)[8], const CvtMfma<DCT> &mt = {},
+                                             int ustride = 72) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
     const tp_task *tk = a.tasks + task_num;
@@ -250,7 +253,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // goes to row li of the staging block, then lane li reads its row li
     // (element k from lane k).
     const int lane = __lane_id();
-    float *stage_c = lds_c - (lane >> 3) * 72, *stage_s = lds_s - (lane >> 3) * 72;
+    float *stage_c = lds_c - (lane >> 3) * ustride, *stage_s = lds_s - (lane >> 3) * ustride;
     if constexpr (DCT == 0) {
         dct4_8(f);
         if (color1) dct4_8(fs);
@@ -261,8 +264,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
             *(float4 *)(lds_s + li * 8 + 4) = make_float4(fs[4], fs[5], fs[6], fs[7]);
         }
     } else {
-        mfma_dct8_to_lds<DCT>(f, mt.c4, stage_c, lane);
-        if (color1) mfma_dct8_to_lds<DCT>(fs, mt.c4, stage_s, lane);
+        mfma_dct8_to_lds<DCT>(f, mt.c4, stage_c, lane, ustride);
+        if (color1) mfma_dct8_to_lds<DCT>(fs, mt.c4, stage_s, lane, ustride);
     }
     wave_lds_sync();
     float cc[8], sc[8];
@@ -279,8 +282,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
         dct4_8(cc);
         if (color1) dst4_8(sc);
     } else {
-        mfma_dct8_to_lds<DCT>(cc, mt.c4, stage_c, lane);
-        if (color1) mfma_dct8_to_lds<DCT>(sc, mt.s4, stage_s, lane);
+        mfma_dct8_to_lds<DCT>(cc, mt.c4, stage_c, lane, ustride);
+        if (color1) mfma_dct8_to_lds<DCT>(sc, mt.s4, stage_s, lane, ustride);
         wave_lds_sync();
 #pragma unroll
         for (int k = 0; k < 8; k++) cc[k] = lds_c[li * 8 + k];

@@ -58,19 +58,19 @@ struct FusedArgs {
 };
 
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>
-__global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_FUSED_WAVES)))
+void k_fused(FusedArgs a) {
     static_assert(GT == 2 || GT == 4, "tiles per group");
     constexpr int NSLOT = 6 * GT;
+    // [tile][camera][color][4 x 8 x 8]; a unit's slice also serves as its
+    // transposition staging (2 x 72 floats) before its CLT is written there
     __shared__ __align__(16) float clt_lds[GT * 4 * 3 * 256];
-    __shared__ __align__(16) float cvt_lds[3][2][8][72];
     __shared__ int s_item;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
     const int li = lane & 7;
     const int cam = unit & 3;
-    float *lds_c = &cvt_lds[wave][0][unit][0];
-    float *lds_s = &cvt_lds[wave][1][unit][0];
     const LaneFold lf = lane_fold(li);
     const CvtMfma<DCT> mt = cvt_mfma<DCT>(lane);
     int src_off[4];
@@ -124,18 +124,15 @@ __global__ __launch_bounds__(192) void k_fused(FusedArgs a) {
                 const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
                 valid = valid && fr.tasks[task_num].task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
-                int color;
-                if (wave == 0) {
-                    convert_unit<0, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
-                    color = 0;
-                } else if (wave == 1) {
-                    convert_unit<1, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
-                    color = 1;
-                } else {
-                    convert_unit<2, DCT>(ca, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3, mt);
-                    color = 2;
-                }
-                float *d = clt_lds + ((tl * 4 + cam) * 3 + color) * 256 + li * 8;
+                const int color = wave;
+                float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * 256;  // units 768 floats apart
+                if (wave == 0)
+                    convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                else if (wave == 1)
+                    convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                else
+                    convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                float *d = slice + li * 8;
 #pragma unroll
                 for (int qq = 0; qq < 4; qq++) {
                     const float *q = qq == 0 ? q0 : qq == 1 ? q1 : qq == 2 ? q2 : q3;

@@ -111,10 +111,10 @@ __device__ __forceinline__ MfmaConst<PREC> mfma_const(int lane) {
 }
 
 // y = T x for the 64 lane vectors of the wavefront; lane (u, c) of the
-// results is stored at stage + u * 72 + c * 8 + m (m = 0..7).
+// results is stored at stage + u * ustride + c * 8 + m (m = 0..7).
 template <int PREC>
 __device__ __forceinline__ void mfma_dct8_to_lds(const float (&x)[8], const MfmaConst<PREC> &k,
-                                                 float *stage, int lane) {
+                                                 float *stage, int lane, int ustride) {
     typedef typename SplitOps<PREC>::vec vec;
     vec xh[2], xl[2];
     float inv = 1.0f;
@@ -156,7 +156,7 @@ __device__ __forceinline__ void mfma_dct8_to_lds(const float (&x)[8], const Mfma
         const int u = src >> 3;
         const int c = lane & 7;
         const int m0 = 4 * (grp & 1);
-        *(f32x4 *)(stage + u * 72 + c * 8 + m0) = d;
+        *(f32x4 *)(stage + u * ustride + c * 8 + m0) = d;
     }
 }
 

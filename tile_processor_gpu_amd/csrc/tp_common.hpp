@@ -25,6 +25,9 @@
 #ifndef TP_CORR_EPB
 #define TP_CORR_EPB 64     // correlation entries per workgroup (multiple of 32)
 #endif
+#ifndef TP_FUSED_WAVES
+#define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
+#endif
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
