@@ -411,21 +411,46 @@ __device__ __forceinline__ void store_clt_rows(float *dst, int li, const float (
     store4(dst + 196, q3[4], q3[5], q3[6], q3[7]);
 }
 
+// One unit through to memory.  slice: the unit's 256-float LDS block (its
+// transposition staging, then its output block).
 template <int COLOR>
 __device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const LaneFold &lf,
-                                                   float *lds_c, float *lds_s, int li, int cam,
-                                                   int task_num, bool valid) {
+                                                   float *slice, int li, int cam, int task_num,
+                                                   bool valid) {
     float q0[8], q1[8], q2[8], q3[8];
-    convert_unit<COLOR>(a, lf, lds_c, lds_s, li, cam, task_num, q0, q1, q2, q3);
+    convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, {}, 256);
     // (12) store [q][l][k], :3600-3630; out-of-grid tiles are skipped
     const int tile = task_tile(a.g, a.tasks[task_num].txy);
-    if (valid && tile >= 0)
-        store_clt_rows(a.clt[cam] + ((size_t)tile * TP_NUM_COLORS + COLOR) * 256, li, q0, q1, q2, q3);
+    const bool st = valid && tile >= 0;
+    float *dst = a.clt[cam] + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
+#if TP_CONVERT_LDS_STORE
+    // The unit's 1 KB block goes through LDS so that store j writes float4
+    // j * 8 + li: every store instruction covers whole 128-byte lines of
+    // each unit (non-temporal stores of partial lines cost extra writes).
+    // The slice was last read before convert_unit's final wavefront fence.
+    {
+        const float *q[4] = {q0, q1, q2, q3};
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            *(float4 *)(slice + qq * 64 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
+            *(float4 *)(slice + qq * 64 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const f32x4 v = *(const f32x4 *)(slice + (j * 8 + li) * 4);
+        if (st) store4(dst + (j * 8 + li) * 4, v[0], v[1], v[2], v[3]);
+    }
+    wave_lds_sync();  // the slice is the next unit's staging
+#else
+    if (st) store_clt_rows(dst, li, q0, q1, q2, q3);
+#endif
 }
 
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_correct_tiles(ConvertArgs a) {
-    __shared__ __align__(16) float lds[3][2][8][72];
+    __shared__ __align__(16) float lds[3][8][256];  // [wave][unit]: staging, then output block
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -435,8 +460,7 @@ void k_convert_correct_tiles(ConvertArgs a) {
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int p0 = blk * a.pairs_per_block;
     const int p1 = min(npairs, p0 + a.pairs_per_block);
-    float *lds_c = &lds[wave][0][unit][0];
-    float *lds_s = &lds[wave][1][unit][0];
+    float *slice = &lds[wave][unit][0];
     const LaneFold lf = lane_fold(li);
     for (int p = p0; p < p1; p++) {
         const int s = p * 2 + (unit >> 2);
@@ -444,11 +468,11 @@ void k_convert_correct_tiles(ConvertArgs a) {
         const int task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
         valid = valid && (a.tasks[task_num].task != 0);
         if (wave == 0)
-            convert_store_unit<0>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<0>(a, lf, slice, li, cam, task_num, valid);
         else if (wave == 1)
-            convert_store_unit<1>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<1>(a, lf, slice, li, cam, task_num, valid);
         else
-            convert_store_unit<2>(a, lf, lds_c, lds_s, li, cam, task_num, valid);
+            convert_store_unit<2>(a, lf, slice, li, cam, task_num, valid);
     }
 }
 

@@ -186,6 +186,9 @@ void k_correlate2D(CorrArgs a) {
     const float fz2 = a.fat_zero * a.fat_zero;
     int src_off[4];
     corr_out_offsets(lane, a.radius, src_off);
+    // rows of radius 7 at stride 225 are contiguous: whole-block stores
+    const bool dense = a.radius == 7 && a.stride == 225 &&
+                       (((uintptr_t)a.corrs) & 15) == 0;
 
     for (int eb = e0 + wave * 8; eb < e1; eb += 32) {
         int e = eb + unit;
@@ -210,6 +213,24 @@ void k_correlate2D(CorrArgs a) {
         corr_finish(c0, c1, c2, c3, L, li, fz2);
         wave_lds_sync();
         const int vflag = valid ? 1 : 0;
+#if TP_CORR_BLOCK_STORE
+        if (dense && __all(vflag)) {
+            // 8 consecutive full 15 x 15 rows, contiguous in memory (7200 B,
+            // 16-byte aligned): float4 q of the block from the staged tiles
+            float *dst = a.corrs + (size_t)eb * 225;
+            for (int q = lane; q < 450; q += 64) {
+                float v[4];
+#pragma unroll
+                for (int e4 = 0; e4 < 4; e4++) {
+                    const int f = 4 * q + e4, u = f / 225;
+                    v[e4] = (&lds[wave][u][0][0])[f - 225 * u];  // staged tile u, element f - 225 u
+                }
+                store4(dst + 4 * q, v[0], v[1], v[2], v[3]);
+            }
+            wave_lds_sync();
+            continue;
+        }
+#endif
         for (int u = 0; u < 8; u++) {
             if (!__shfl(vflag, u * 8)) continue;
             const float *src = &lds[wave][u][0][0];

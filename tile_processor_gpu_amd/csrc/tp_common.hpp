@@ -31,6 +31,12 @@
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
+#ifndef TP_CONVERT_LDS_STORE
+#define TP_CONVERT_LDS_STORE 1  // forward CLT output through LDS as whole 128-byte lines
+#endif
+#ifndef TP_CORR_BLOCK_STORE
+#define TP_CORR_BLOCK_STORE 1  // correlation rows of radius 7 written as contiguous 8-row blocks
+#endif
 #ifndef TP_NT_STORE
 #define TP_NT_STORE 1      // write-once outputs bypass L2 retention (non-temporal)
 #endif
