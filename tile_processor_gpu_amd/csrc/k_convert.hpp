@@ -79,6 +79,36 @@ __device__ __forceinline__ int swizzle_strip(int s, int TX, int num_tiles) {
     return st * strip + r * TX + c;
 }
 
+// Panel order for a full raster task list (num_tiles == TX * TY): panels of
+// PW tile columns, each visited as swizzle_strip visits the whole grid
+// (strips of TP_STRIP_ROWS rows, column pairs top to bottom), so that the
+// kernel cells a strip shares with the next one are still in L2 (a panel's
+// row of cells is (PW / 2 + 1) x 12 KB).  A bijection on [0, TX * TY).
+// Arguments are wavefront-uniform, so it runs on the scalar unit.
+__device__ __forceinline__ int swizzle_panel(int s, int TX, int TY, int PW) {
+    constexpr int SH = TP_STRIP_ROWS;
+    const int nfull = TX / PW, per = PW * TY;
+    int p = s / per, w = PW;
+    if (p >= nfull) {
+        p = nfull;
+        w = TX - nfull * PW;
+    }
+    const int q = s - p * per;
+    const int st = q / (SH * w), r = q - st * SH * w;
+    const int h = min(SH, TY - st * SH);
+    const int full = (w >> 1) * 2 * h;
+    int row, x;
+    if (r < full) {
+        const int cp = r / (2 * h), sub = r - cp * 2 * h;
+        row = sub >> 1;
+        x = 2 * cp + (sub & 1);
+    } else {
+        row = r - full;
+        x = w - 1;
+    }
+    return (st * SH + row) * TX + p * PW + x;
+}
+
 // Per-lane constants of the horizontal fold: folded column li is fed by
 // source columns ca (the smaller) and cb; for each local column parity p:
 // the column and its window coefficients (w_s(c) = P cos(pi s/16) +
@@ -208,6 +238,15 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // in the row; windows reaching past the right/bottom edge read the
     // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
     gfloat *img = gptr(a.images[cam]);
+    // pixel loads; with TP_IMG_NT they are non-temporal (evict-first in L2),
+    // leaving the aberration kernels, which 4 tiles share, resident longer
+    auto pix = [&](int o) -> float {
+#if TP_IMG_NT
+        return __builtin_nontemporal_load(img + o);
+#else
+        return img[o];
+#endif
+    };
     const int ds = (int)a.dstride;
     const bool inside = (itlx + 15 < g.W) && (itly + 15 < g.H);
     auto rowoff = [&](int r) -> int {
@@ -233,7 +272,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
             const bool use_a = (ra & 1) == pr;
             const int r = use_a ? ra : rb;
             const float w = use_a ? wv[ra] : wv[rb];
-            f[l] = img[rowoff(r) + xo] * w * wc;
+            f[l] = pix(rowoff(r) + xo) * w * wc;
         }
     } else {
         // G sites: (row + col + itlx + itly) even; both source rows used
@@ -242,8 +281,8 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
             const int pca = (ra + S) & 1;  // column parity for row ra; row rb takes the other
-            const float da = img[rowoff(ra) + (pca ? x1 : x0)] * wv[ra];
-            const float db = img[rowoff(rb) + (pca ? x0 : x1)] * wv[rb];
+            const float da = pix(rowoff(ra) + (pca ? x1 : x0)) * wv[ra];
+            const float db = pix(rowoff(rb) + (pca ? x0 : x1)) * wv[rb];
             f[l] = fmaf(da, pca ? whc1 : whc0, db * (pca ? whc0 : whc1));
             fs[l] = fmaf(da, pca ? whs1 : whs0, db * (pca ? whs0 : whs1));
         }
@@ -462,10 +501,20 @@ void k_convert_correct_tiles(ConvertArgs a) {
     const int p1 = min(npairs, p0 + a.pairs_per_block);
     float *slice = &lds[wave][unit][0];
     const LaneFold lf = lane_fold(li);
+    const bool panels = TP_PANEL_W > 0 && a.num_tiles == a.g.TX * a.g.TY;
     for (int p = p0; p < p1; p++) {
         const int s = p * 2 + (unit >> 2);
         bool valid = s < a.num_tiles;
-        const int task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
+        int task_num;
+        if (panels) {  // the pair's two tiles from uniform values, then per lane
+            const int sp = __builtin_amdgcn_readfirstlane(p * 2);
+            const int t0 = swizzle_panel(sp, a.g.TX, a.g.TY, TP_PANEL_W);
+            const int t1 = sp + 1 < a.num_tiles ? swizzle_panel(sp + 1, a.g.TX, a.g.TY, TP_PANEL_W) : t0;
+            task_num = (unit >> 2) ? t1 : t0;
+            if (!valid) task_num = a.num_tiles - 1;
+        } else {
+            task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
+        }
         valid = valid && (a.tasks[task_num].task != 0);
         if (wave == 0)
             convert_store_unit<0>(a, lf, slice, li, cam, task_num, valid);

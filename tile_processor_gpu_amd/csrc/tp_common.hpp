@@ -31,6 +31,12 @@
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
+#ifndef TP_PANEL_W
+#define TP_PANEL_W 0       // forward CLT visiting order: panel width in tiles (0: full-width strips)
+#endif
+#ifndef TP_IMG_NT
+#define TP_IMG_NT 0        // forward CLT: non-temporal pixel loads
+#endif
 #ifndef TP_CONVERT_LDS_STORE
 #define TP_CONVERT_LDS_STORE 1  // forward CLT output through LDS as whole 128-byte lines
 #endif
