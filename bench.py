@@ -279,12 +279,15 @@ def measure_stages(cfg, fb, idx, reps):
         fb.fused(3, SCALES, FAT_ZERO)
         x = torch.empty((g.num_tiles, 7, 225), device=dev)
 
-        def nn_frame():
+        def nn_input():
             _lib.call("tp_nn_input", ctypes.byref(g.c()), ctypes.c_void_p(fb.tasks.data_ptr()),
                       g.num_tiles, ctypes.c_void_p(fb.corrs.data_ptr()), fb.corr_stride,
                       ctypes.c_void_p(x.data_ptr()), tp._stream(None))
-            model.forward(x, g.num_tiles)
-        out["nn_frame_ms"] = round(median_ms(nn_frame, max(3, reps // 4), warmup=1), 4)
+        out["nn_input_ms"] = round(median_ms(nn_input, max(3, reps // 4), warmup=1), 4)
+        out["nn_frame_ms_matrix_core_bf16"] = round(
+            median_ms(lambda: model.infer(x, g.num_tiles), max(3, reps // 4), warmup=1), 4)
+        out["nn_frame_ms_fp32"] = round(
+            median_ms(lambda: model.forward(x, g.num_tiles), 3, warmup=1), 4)
         del x
         res = nn.evaluate(model, nn.TileSource(4, seed=777), batches=2)
         out["nn_accuracy"] = {"data": "rendered piecewise-planar scenes, 4 x 256 x 256, "

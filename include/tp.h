@@ -451,6 +451,11 @@ int tp_nn_input(const tp_geometry *geom, const tp_task *gpu_tasks, int num_tasks
 int tp_nn_forward(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_acts,
                   float *gpu_out, tp_stream_t stream);
 
+/* Inference form of tp_nn_forward: BF16 activations, the two 32-channel
+ * convolutions on the matrix cores (FP32 accumulation). */
+int tp_nn_infer(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_out,
+                tp_stream_t stream);
+
 /* Adds the gradient of the mean training loss over the tiles with a finite
  * target (huber(d - target) + conf_weight * BCE(confidence, |d - target| <
  * 0.5)) to gpu_grads[TP_NN_PARAMS] and the loss to gpu_loss[0];

@@ -297,7 +297,7 @@ def bf16_round(a):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def nn_forward(prm, x, with_acts=False):
+def nn_forward(prm, x, with_acts=False, bf16_acts=False):
     """prm: [11874] parameters (float64; pass the BF16-rounded values to
     mirror the GPU); x: [n][7][225] -> out [n][2] (disparity, confidence)
     and optionally acts [n][14850]."""
@@ -306,8 +306,9 @@ def nn_forward(prm, x, with_acts=False):
     n = x.shape[0]
     out = np.zeros((n, 2))
     acts = np.zeros((n, NN_ACTS)) if with_acts else None
-    lib().oracle_nn_forward(prm.ctypes.data_as(_VP), x.ctypes.data_as(_VP), ctypes.c_int(n),
-                            acts.ctypes.data_as(_VP) if with_acts else _VP(0), out.ctypes.data_as(_VP))
+    lib().oracle_nn_forward_ex(prm.ctypes.data_as(_VP), x.ctypes.data_as(_VP), ctypes.c_int(n),
+                               acts.ctypes.data_as(_VP) if with_acts else _VP(0),
+                               out.ctypes.data_as(_VP), ctypes.c_int(int(bf16_acts)))
     return (out, acts) if with_acts else out
 
 

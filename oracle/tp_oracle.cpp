@@ -1159,14 +1159,34 @@ static void nn_head(const double *z, double *prob, double &disp, double &cl) {
     cl = sc;
 }
 
-/* out[n][2] = (disparity, confidence); acts[n][14850] = h1, h2, z (or NULL) */
-void oracle_nn_forward(const double *prm, const float *x, int n, double *acts, double *out) {
+/* BF16 (round to nearest even) of a value first rounded to float */
+static double bf16_of(double v) {
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+/* out[n][2] = (disparity, confidence); acts[n][14850] = h1, h2, z (or NULL);
+ * bf16_acts: the input and both hidden layers rounded to BF16 before they
+ * enter the next convolution (the matrix-core inference kernel's precision) */
+void oracle_nn_forward_ex(const double *prm, const float *x, int n, double *acts, double *out,
+                          int bf16_acts) {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int t = 0; t < n; t++) {
         std::vector<double> in(7 * NN_P), h1(32 * NN_P), h2(32 * NN_P), z(2 * NN_P), prob(NN_P);
         for (int i = 0; i < 7 * NN_P; i++) in[i] = x[(size_t)t * 7 * NN_P + i];
+        if (bf16_acts)
+            for (auto &v : in) v = bf16_of(v);
         nn_conv(in.data(), 7, prm + 0, prm + 2016, 32, true, h1.data());
+        if (bf16_acts)
+            for (auto &v : h1) v = bf16_of(v);
         nn_conv(h1.data(), 32, prm + 2048, prm + 11264, 32, true, h2.data());
+        if (bf16_acts)
+            for (auto &v : h2) v = bf16_of(v);
         nn_conv(h2.data(), 32, prm + 11296, prm + 11872, 2, false, z.data());
         double disp, cl;
         nn_head(z.data(), prob.data(), disp, cl);
@@ -1179,6 +1199,10 @@ void oracle_nn_forward(const double *prm, const float *x, int n, double *acts, d
             std::copy(z.begin(), z.end(), a + 64 * NN_P);
         }
     }
+}
+
+void oracle_nn_forward(const double *prm, const float *x, int n, double *acts, double *out) {
+    oracle_nn_forward_ex(prm, x, n, acts, out, 0);
 }
 
 /* Gradient of sum over tiles with a finite target of scale * (huber_delta(d

@@ -77,6 +77,23 @@ def test_nn_forward_matches_oracle(orc, net_case):
     assert torch.equal(out, out2)
 
 
+def test_nn_infer_matrix_core_matches_oracle(orc, net_case):
+    """Matrix-core inference (BF16 activations) against the oracle with the
+    activations rounded to BF16 at the same points: disparity within 0.02 px
+    and confidence within 0.01 (a BF16 rounding that falls on the other side
+    of a tie moves one activation by 2^-8 relative)."""
+    x, n, model, prm = net_case["x"], net_case["n"], net_case["model"], net_case["prm"]
+    out = model.infer(x, n)
+    torch.cuda.synchronize()
+    ref = orc.nn_forward(prm, x.cpu().numpy(), bf16_acts=True)
+    got = out.cpu().numpy()
+    assert np.abs(got[:, 0] - ref[:, 0]).max() <= 0.02, np.abs(got[:, 0] - ref[:, 0]).max()
+    assert np.abs(got[:, 1] - ref[:, 1]).max() <= 0.01, np.abs(got[:, 1] - ref[:, 1]).max()
+    full = orc.nn_forward(prm, x.cpu().numpy())  # FP32-activation network, for scale
+    print(f"[gpu nn] infer vs bf16 oracle {np.abs(got[:, 0] - ref[:, 0]).max():.2e} px, "
+          f"bf16 vs fp activations {np.abs(ref[:, 0] - full[:, 0]).max():.2e} px", flush=True)
+
+
 def test_nn_backward_matches_oracle(orc, net_case):
     x, n, model, prm = net_case["x"], net_case["n"], net_case["model"], net_case["prm"]
     disp = orc.nn_forward(prm, x.cpu().numpy())[:, 0]
@@ -128,4 +145,5 @@ def test_trained_network_beats_centroid():
     res = nn.evaluate(model, nn.TileSource(4, seed=777), batches=2)
     print(f"[gpu nn] {res}", flush=True)
     assert res["nn"]["rmse"] <= res["centroid"]["rmse"]
+    assert res["nn_matrix_core_bf16"]["rmse"] <= res["centroid"]["rmse"]
     assert res["nn_with_fallback"]["rmse"] <= res["centroid"]["rmse"]

@@ -399,6 +399,178 @@ __global__ __launch_bounds__(256) void k_nn_backward(NNBackArgs a) {
     if (t < 2) atomicAdd(&a.grads[NN_OFF_B3 + t], gb3);
 }
 
+// ----------------------------------------------------- inference (MFMA)
+// The same network with BF16 activations, the two 32-channel convolutions
+// as implicit GEMMs on v_mfma_f32_32x32x16_bf16 (D[co][pixel] = sum_k
+// W[co][k] X[k][pixel], k = (3x3 neighbour r, input channel)), FP32
+// accumulation; the 2-channel output convolution and the head in FP32 on
+// the vector units.
+//
+// Activations live in LDS channel-last on a zero-padded 17 x 17 grid
+// (pixel (y, x) at (y + 1) * 17 + x + 1), so the 8 consecutive channels a
+// lane contributes to one K slice are one 16-byte load at any neighbour.
+// Operand layouts (32x32x16): lane i holds A[i % 32][8 (i / 32) + e],
+// B[8 (i / 32) + e][i % 32] (e = 0..7) and D[8 (v / 4) + 4 (i / 32) + v % 4]
+// [i % 32] (v = 0..15).  Each wavefront takes 2 of a tile's 8 blocks of 32
+// pixels; the weights of both convolutions stay in registers.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int NN_PAD = 17, NN_PADP = NN_PAD * NN_PAD;
+
+__device__ __forceinline__ int nn_padded(int p) { return (p / 15 + 1) * NN_PAD + p % 15 + 1; }
+
+// A fragment: weights W[co][ci0 + e][r] of layer (CIN inputs), e = 0..7
+// (channels >= cin_valid or r >= 9 give 0)
+template <int CIN>
+__device__ __forceinline__ u32x4 nn_wfrag(const uint16_t *__restrict__ w, int co, int ci0, int r,
+                                          int cin_valid) {
+    uint32_t h[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int ci = ci0 + e;
+        h[e] = (r < 9 && ci < cin_valid) ? (uint32_t)w[(co * CIN + ci) * 9 + r] : 0u;
+    }
+    u32x4 v;
+#pragma unroll
+    for (int d = 0; d < 4; d++) v[d] = h[2 * d] | (h[2 * d + 1] << 16);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t bf16pair(float a, float b) {
+    return (uint32_t)f2bf16(a) | ((uint32_t)f2bf16(b) << 16);
+}
+
+// one 32-pixel block of a 3x3 convolution: in = LDS activations with C
+// channels per pixel; NK chunks of 16 K values; A = weight fragments.
+// k ordering per chunk kc: C = 8: r = 2 kc + i / 32, channels 0..7;
+// C = 32: r = kc / 2, channels (kc % 2) 16 + 8 (i / 32) .. + 7.
+template <int C, int NK>
+__device__ __forceinline__ f32x16 nn_conv_block(const uint16_t *in, const u32x4 (&A)[NK], int p,
+                                               int lane, f32x16 acc) {
+    const bool valid = p < NN_P;
+    const int pp = valid ? nn_padded(p) : 0;  // ring pixel 0 is zero
+    const int half = lane >> 5;
+#pragma unroll
+    for (int kc = 0; kc < NK; kc++) {
+        int r, c0;
+        if (C == 8) {
+            r = 2 * kc + half;
+            c0 = 0;
+        } else {
+            r = kc >> 1;
+            c0 = (kc & 1) * 16 + 8 * half;
+        }
+        u32x4 b = {0u, 0u, 0u, 0u};
+        if (r < 9) {
+            const int q = valid ? pp + (r / 3 - 1) * NN_PAD + (r % 3 - 1) : 0;
+            b = *(const u32x4 *)(in + q * C + c0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, A[kc]),
+                                                      __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// bias + ReLU, BF16, channel-last store of this lane's 16 channels of pixel p
+__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, int p, int lane) {
+    if (p >= NN_P) return;
+    uint16_t *o = out + nn_padded(p) * NN_C + 4 * (lane >> 5);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t lo = bf16pair(fmaxf(acc[4 * j], 0.f), fmaxf(acc[4 * j + 1], 0.f));
+        const uint32_t hi = bf16pair(fmaxf(acc[4 * j + 2], 0.f), fmaxf(acc[4 * j + 3], 0.f));
+        *(uint2 *)(o + 8 * j) = make_uint2(lo, hi);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nn_infer(const uint16_t *__restrict__ prm,
+                                                  const float *__restrict__ x, int n, float *out) {
+    __shared__ __align__(16) uint16_t X0[NN_PADP * 8];
+    __shared__ __align__(16) uint16_t H1[NN_PADP * NN_C];
+    __shared__ __align__(16) uint16_t H2[NN_PADP * NN_C];
+    __shared__ __align__(16) float W3s[2 * 9 * NN_C];  // [co][r][ci]
+    __shared__ float red[16];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int co = lane & 31, half = lane >> 5;
+    for (int i = t; i < NN_PADP * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
+    for (int i = t; i < NN_PADP * 16; i += 256) {
+        ((uint32_t *)H1)[i] = 0u;
+        ((uint32_t *)H2)[i] = 0u;
+    }
+    for (int i = t; i < 576; i += 256) {
+        const int c = i / 288, r = (i / 32) % 9, ci = i % 32;
+        W3s[i] = bf16f(prm[NN_OFF_W3 + (c * NN_C + ci) * 9 + r]);
+    }
+    u32x4 A1[5], A2[18];
+#pragma unroll
+    for (int kc = 0; kc < 5; kc++) A1[kc] = nn_wfrag<NN_IN>(prm + NN_OFF_W1, co, 0, 2 * kc + half, NN_IN);
+#pragma unroll
+    for (int kc = 0; kc < 18; kc++)
+        A2[kc] = nn_wfrag<NN_C>(prm + NN_OFF_W2, co, (kc & 1) * 16 + 8 * half, kc >> 1, NN_C);
+    f32x16 bias1, bias2;  // accumulator start values: bias of each row this lane holds
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int row = 8 * (v >> 2) + 4 * half + (v & 3);
+        bias1[v] = bf16f(prm[NN_OFF_B1 + row]);
+        bias2[v] = bf16f(prm[NN_OFF_B2 + row]);
+    }
+    const float b30 = bf16f(prm[NN_OFF_B3]), b31 = bf16f(prm[NN_OFF_B3 + 1]);
+    __syncthreads();
+    for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
+        const float *xt = x + (size_t)tile * NN_IN * NN_P;
+        for (int i = t; i < NN_IN * NN_P; i += 256) {
+            const int c = i / NN_P, p = i - c * NN_P;
+            X0[nn_padded(p) * 8 + c] = f2bf16(xt[i]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+            const int p = 32 * (2 * wave + bb) + co;
+            nn_store_block(H1, nn_conv_block<8, 5>(X0, A1, p, lane, bias1), p, lane);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+            const int p = 32 * (2 * wave + bb) + co;
+            nn_store_block(H2, nn_conv_block<NN_C, 18>(H1, A2, p, lane, bias2), p, lane);
+        }
+        __syncthreads();
+        float z0 = b30, z1 = b31;
+        const bool px = t < NN_P;
+        if (px) {
+            const int pp = nn_padded(t);
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                const int q = pp + (r / 3 - 1) * NN_PAD + (r % 3 - 1);
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    const u32x4 hv = *(const u32x4 *)(H2 + q * NN_C + 8 * cb);
+                    const float4 *w0 = (const float4 *)(W3s + (0 * 9 + r) * NN_C + 8 * cb);
+                    const float4 *w1 = (const float4 *)(W3s + (1 * 9 + r) * NN_C + 8 * cb);
+                    const float4 a0 = w0[0], a1 = w0[1], c0 = w1[0], c1 = w1[1];
+                    const float h[8] = {__uint_as_float(hv[0] << 16), __uint_as_float(hv[0] & 0xffff0000u),
+                                        __uint_as_float(hv[1] << 16), __uint_as_float(hv[1] & 0xffff0000u),
+                                        __uint_as_float(hv[2] << 16), __uint_as_float(hv[2] & 0xffff0000u),
+                                        __uint_as_float(hv[3] << 16), __uint_as_float(hv[3] & 0xffff0000u)};
+                    z0 = fmaf(a0.x, h[0], fmaf(a0.y, h[1], fmaf(a0.z, h[2], fmaf(a0.w, h[3], z0))));
+                    z0 = fmaf(a1.x, h[4], fmaf(a1.y, h[5], fmaf(a1.z, h[6], fmaf(a1.w, h[7], z0))));
+                    z1 = fmaf(c0.x, h[0], fmaf(c0.y, h[1], fmaf(c0.z, h[2], fmaf(c0.w, h[3], z1))));
+                    z1 = fmaf(c1.x, h[4], fmaf(c1.y, h[5], fmaf(c1.z, h[6], fmaf(c1.w, h[7], z1))));
+                }
+            }
+        }
+        float prob, disp, cl;
+        head_stats(z0, z1, px, px ? t % 15 : 0, red, prob, disp, cl);
+        if (t == 0) {
+            out[2 * tile] = disp;
+            out[2 * tile + 1] = 1.0f / (1.0f + __expf(-cl));
+        }
+        __syncthreads();  // X0, H1, H2 and red are rewritten by the next tile
+    }
+}
+
 // ---------------------------------------------------------------- Adam
 __global__ void k_nn_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1,
                           float b2, float eps, float c1, float c2, uint16_t *wb) {

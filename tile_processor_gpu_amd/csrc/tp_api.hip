@@ -651,6 +651,17 @@ int tp_nn_forward(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, fl
     return check_launch("nn_forward");
 }
 
+int tp_nn_infer(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_out,
+                tp_stream_t stream) {
+    if (!gpu_params_bf16 || !gpu_x || !gpu_out) return fail(TP_EINVAL, "tp_nn_infer: NULL pointer argument%s", "");
+    if (n < 0) return fail(TP_EINVAL, "tp_nn_infer: n < 0%s", "");
+    g_err.clear();
+    if (!n) return TP_OK;
+    hipLaunchKernelGGL(tp::k_nn_infer, dim3(nn_grid(n)), dim3(256), 0, as_stream(stream),
+                       gpu_params_bf16, gpu_x, n, gpu_out);
+    return check_launch("nn_infer");
+}
+
 int tp_nn_backward(const uint16_t *gpu_params_bf16, const float *gpu_x, const float *gpu_acts,
                    const float *gpu_target, int n, float conf_weight, float huber_delta,
                    float loss_scale, float *gpu_grads, float *gpu_loss, tp_stream_t stream) {

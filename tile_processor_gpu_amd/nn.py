@@ -96,6 +96,14 @@ class DisparityCNN:
         _lib.call("tp_nn_forward", _p(self.bf16), _p(x), int(n), _p(acts), _p(out), tp._stream(stream))
         return out
 
+    def infer(self, x, n, out=None, stream=None):
+        """Inference on the matrix cores (BF16 activations): x [n][7][225] ->
+        out [n][2] (disparity, confidence)."""
+        if out is None:
+            out = torch.empty((n, 2), dtype=torch.float32, device=self.device)
+        _lib.call("tp_nn_infer", _p(self.bf16), _p(x), int(n), _p(out), tp._stream(stream))
+        return out
+
     def gradient(self, x, target, n, acts, conf_weight=0.1, huber_delta=1.0, stream=None):
         """Gradient of the mean loss into self.grads (overwritten); returns
         the loss tensor (device, not synchronised)."""
@@ -170,16 +178,18 @@ def metrics(est, target, conf=None, fallback=None, threshold=0.5):
 
 def evaluate(model, source, batches=2):
     """Network vs centroid on fresh frames from `source`."""
-    nn_est, nn_conf, cen, tg = [], [], [], []
+    nn_est, nn_conf, nn_bf, cen, tg = [], [], [], [], []
     for _ in range(batches):
         x, target, n = source.next()
         out = model.forward(x, n)
+        out_bf = model.infer(x, n)
         torch.cuda.synchronize()
         o = out.cpu().numpy()
-        nn_est.append(o[:, 0]); nn_conf.append(o[:, 1])
+        nn_est.append(o[:, 0]); nn_conf.append(o[:, 1]); nn_bf.append(out_bf[:, 0].cpu().numpy())
         cen.append(source.baseline[:, 0].cpu().numpy()); tg.append(target.cpu().numpy())
-    nn_est, nn_conf, cen, tg = map(np.concatenate, (nn_est, nn_conf, cen, tg))
-    return {"nn": metrics(nn_est, tg), "centroid": metrics(cen, tg),
+    nn_est, nn_conf, nn_bf, cen, tg = map(np.concatenate, (nn_est, nn_conf, nn_bf, cen, tg))
+    return {"nn": metrics(nn_est, tg), "nn_matrix_core_bf16": metrics(nn_bf, tg),
+            "centroid": metrics(cen, tg),
             "nn_with_fallback": metrics(nn_est, tg, nn_conf, cen),
             "mean_confidence": float(np.nanmean(nn_conf))}
 
