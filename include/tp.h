@@ -341,7 +341,8 @@ typedef struct tp_fused_config {
 /* One frame of the stream form; an array of these lives in device memory.
  * images / kernels / kernel_offsets / clt are device arrays of TP_NUM_CAMS
  * device pointers as for tp_convert_correct_tiles; clt may be NULL (CLT not
- * stored).  item_start is written by the library. */
+ * stored), corrs may be NULL (pair tiles not stored), combined may be NULL
+ * (no six-pair combination).  item_start is written by the library. */
 typedef struct tp_frame {
     const tp_task *tasks;
     float **images;
@@ -349,12 +350,16 @@ typedef struct tp_frame {
     float **kernel_offsets;
     float **clt;
     float *corrs;
+    float *combined;
     int32_t num_tiles;
     int32_t item_start;
 } tp_frame;
 
 /* Forward CLT + correlation of one frame.  Arguments as for
- * tp_convert_correct_tiles and tp_correlate2D; gpu_clt may be NULL. */
+ * tp_convert_correct_tiles and tp_correlate2D; gpu_clt and gpu_corrs may be
+ * NULL.  gpu_combined (or NULL): row `task` receives the combination of the
+ * task's pairs in pair_mask, as tp_corr_combine computes it
+ * (combined_stride floats per row). */
 int tp_convert_correlate(const tp_geometry *geom,
                          float **gpu_kernel_offsets,
                          float **gpu_kernels,
@@ -372,6 +377,9 @@ int tp_convert_correlate(const tp_geometry *geom,
                          size_t corr_stride,
                          int corr_radius,
                          float *gpu_corrs,
+                         float *gpu_combined,
+                         size_t combined_stride,
+                         int pair_mask,
                          const tp_fused_config *config,
                          tp_stream_t stream);
 
@@ -395,6 +403,8 @@ int tp_stream_convert_correlate(const tp_geometry *geom,
                                 float fat_zero,
                                 size_t corr_stride,
                                 int corr_radius,
+                                size_t combined_stride,
+                                int pair_mask,
                                 const tp_fused_config *config,
                                 tp_stream_t stream);
 
@@ -417,6 +427,8 @@ int tp_stream_graph_create(const tp_geometry *geom,
                            float fat_zero,
                            size_t corr_stride,
                            int corr_radius,
+                           size_t combined_stride,
+                           int pair_mask,
                            const tp_fused_config *config,
                            tp_graph_t *graph);
 int tp_graph_launch(tp_graph_t graph, tp_stream_t stream);

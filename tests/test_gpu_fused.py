@@ -184,3 +184,42 @@ def test_stream_graph_replays(stream):
         for i, fb in enumerate(fbs):
             assert torch.equal(got[i], fb.corrs[:6 * fb.num_tiles]), (rep, i)
     run.release()
+
+
+@pytest.mark.parametrize("pair_mask", [0x3F, 0x13])
+def test_fused_combined_matches_combine(small, orc, pair_mask):
+    """The six-pair combination written by the fused kernel (pair rows not
+    stored) against tp_corr_combine on the staged correlations and against
+    the oracle's combination of the FP64 correlations."""
+    fb, cfg = small["fb"], small["cfg"]
+    g = cfg["geom"]
+    ref_rows = small["corr"]
+    comb_ref = torch.zeros((g.num_tiles, 225), device="cuda:0")
+    tp.corr_combine(g, fb.num_corr, fb.corr_idx, ref_rows, 225, 7, pair_mask, comb_ref, 225)
+    fb.corrs.fill_(SENTINEL)
+    fb.combined_buffer().fill_(SENTINEL)
+    fb.fused(3, SCALES, FZ, store_pairs=False, combined=True, pair_mask=pair_mask)
+    torch.cuda.synchronize()
+    got = fb.combined[:g.num_tiles].cpu().numpy()
+    assert (fb.corrs.cpu().numpy() == SENTINEL).all()  # pair rows not written
+    assert rel_err(got, comb_ref.cpu().numpy()) <= TOL_STAGED
+    clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], cfg["tasks"])
+    c64 = orc.correlate2D(g, clt64, 3, SCALES, FZ, corr_indices(g, cfg["tasks"]), 7)
+    o = orc.corr_combine(c64, 7, pair_mask=pair_mask)
+    assert rel_err(got, o) < TOL
+
+
+def test_stream_combined(stream):
+    fbs = stream["fbs"][:4]
+    refs = []
+    for fb in fbs:
+        fb.fused(3, SCALES, FZ, store_pairs=False, combined=True)
+        torch.cuda.synchronize()
+        refs.append(fb.combined[:fb.num_tiles].clone())
+        fb.combined.fill_(SENTINEL)
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=8), store_pairs=False,
+                       combined=True)
+    run.launch()
+    torch.cuda.synchronize()
+    for i, fb in enumerate(fbs):
+        assert rel_err(fb.combined[:fb.num_tiles].cpu().numpy(), refs[i].cpu().numpy()) <= TOL_STAGED

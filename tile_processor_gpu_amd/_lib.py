@@ -35,10 +35,11 @@ PROTOTYPES = {
     "tp_generate_RBGA": [_G, _P, _I, _P, _P, _P, _I, _I, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F,
                          _F, _F, _I, _I, _SZ, _P, _P],
     "tp_convert_correlate": [_G, _P, _P, _P, _P, _P, _SZ, _I, _I, _I, _F, _F, _F, _F, _SZ, _I, _P,
-                             _P, _P],
-    "tp_stream_convert_correlate": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _P,
-                                    _P],
-    "tp_stream_graph_create": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _P, _P],
+                             _P, _SZ, _I, _P, _P],
+    "tp_stream_convert_correlate": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _SZ,
+                                    _I, _P, _P],
+    "tp_stream_graph_create": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _SZ, _I,
+                               _P, _P],
     "tp_graph_launch": [_P, _P],
     "tp_graph_destroy": [_P],
     "tp_wall_clock_hz": [],

@@ -26,6 +26,7 @@
 
 #include "k_convert.hpp"
 #include "k_correlate.hpp"
+#include "k_combine.hpp"
 
 namespace tp {
 
@@ -36,7 +37,8 @@ struct FrameDesc {
     const float *const *kernels;
     const float *const *koffs;
     float *const *clt;  // may be null: CLT not stored
-    float *corrs;       // row 6 * task + pair
+    float *corrs;       // row 6 * task + pair, may be null
+    float *combined;    // row task: the six-pair combination, may be null
     int num_tiles;
     int item_start;
 };
@@ -55,6 +57,8 @@ struct FusedArgs {
     float fat_zero;
     size_t corr_stride;
     int radius;
+    size_t combined_stride;
+    int pair_mask;            // pairs summed into the combined tile
 };
 
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>
@@ -177,14 +181,40 @@ void k_fused(FusedArgs a) {
                 float *L = clt_lds + slot * 288;
                 corr_finish(c0, c1, c2, c3, L, li, fz2);
                 wave_lds_sync();
-                for (int u = 0; u < 8; u++) {
-                    const int r = __shfl(row, u * 8);
-                    if (r < 0) continue;
-                    const float *src = clt_lds + (wave * 8 + u) * 288;
-                    float *dst = fr.corrs + (size_t)r * a.corr_stride;
+                if (fr.corrs) {
+                    for (int u = 0; u < 8; u++) {
+                        const int r = __shfl(row, u * 8);
+                        if (r < 0) continue;
+                        const float *src = clt_lds + (wave * 8 + u) * 288;
+                        float *dst = fr.corrs + (size_t)r * a.corr_stride;
 #pragma unroll
-                    for (int t = 0; t < 4; t++)
-                        if (src_off[t] >= 0) store1(dst + lane + 64 * t, src[src_off[t]]);
+                        for (int t = 0; t < 4; t++)
+                            if (src_off[t] >= 0) store1(dst + lane + 64 * t, src[src_off[t]]);
+                    }
+                }
+            }
+            // (4) six-pair combination from the staged pair tiles (the
+            // mapping of tp_corr_combine, pairs of the task in pair_mask)
+            if (fr.combined) {
+                __syncthreads();
+                const int r = a.radius, n = 2 * r + 1;
+                for (int o = threadIdx.x; o < GT * n * n; o += blockDim.x) {
+                    const int tl = o / (n * n), oo = o - tl * n * n;
+                    const int s = gs + tl;
+                    if (s >= s1) continue;
+                    const int task_num = swizzle_strip(s, g.TX, fr.num_tiles);
+                    const tp_task *tk = fr.tasks + task_num;
+                    if (tk->task == 0 || task_tile(g, tk->txy) < 0) continue;
+                    const int cy = oo / n - r, cx = oo % n - r;
+                    float sum = 0.f;
+#pragma unroll
+                    for (int pr = 0; pr < TP_NUM_PAIRS; pr++) {
+                        int sy, sx;
+                        if (((a.pair_mask >> pr) & 1) && ((tk->task >> (TP_TASK_CORR_BITS + pr)) & 1) &&
+                            pair_src(pr, r, cy, cx, sy, sx))
+                            sum += clt_lds[(tl * 6 + pr) * 288 + (sy + 7) * 15 + sx + 7];
+                    }
+                    store1(fr.combined + (size_t)task_num * a.combined_stride + oo, sum);
                 }
             }
             __syncthreads();

@@ -87,7 +87,8 @@ int fused_common(const char *fn, const tp_geometry *geom, size_t dstride, int lp
 }
 
 tp::FusedArgs fused_args(const tp_geometry *geom, size_t dstride, int lpf_mask, float s0, float s1,
-                         float s2, float fat_zero, size_t corr_stride, int corr_radius, int tpi) {
+                         float s2, float fat_zero, size_t corr_stride, int corr_radius, int tpi,
+                         size_t combined_stride, int pair_mask) {
     tp::FusedArgs a = {};
     a.g = tp::make_geo(*geom);
     a.tiles_per_item = tpi;
@@ -99,6 +100,8 @@ tp::FusedArgs fused_args(const tp_geometry *geom, size_t dstride, int lpf_mask, 
     a.fat_zero = fat_zero;
     a.corr_stride = corr_stride;
     a.radius = corr_radius;
+    a.combined_stride = combined_stride;
+    a.pair_mask = pair_mask;
     return a;
 }
 
@@ -132,9 +135,12 @@ int num_cus() {
 int enqueue_stream(const tp_geometry *geom, tp_frame *gpu_frames, int nframes, int32_t *gpu_work,
                    uint64_t *gpu_stamps, size_t dstride, int lpf_mask, int colors, float s0,
                    float s1, float s2, float fat_zero, size_t corr_stride, int corr_radius,
-                   const tp_fused_config *config, hipStream_t st, const char *fn) {
+                   size_t combined_stride, int pair_mask, const tp_fused_config *config,
+                   hipStream_t st, const char *fn) {
     int rc = fused_common(fn, geom, dstride, lpf_mask, colors, corr_stride, corr_radius);
     if (rc) return rc;
+    const size_t nn = 2 * corr_radius + 1;
+    if (combined_stride < nn * nn) return fail(TP_EINVAL, "%s: combined_stride < (2r+1)^2", fn);
     if (!gpu_frames || !gpu_work) return fail(TP_EINVAL, "%s: NULL pointer argument", fn);
     if (nframes <= 0) return fail(TP_EINVAL, "%s: nframes must be positive", fn);
     FusedCfg c;
@@ -142,7 +148,7 @@ int enqueue_stream(const tp_geometry *geom, tp_frame *gpu_frames, int nframes, i
     const int cus = num_cus();
     if (cus <= 0) return fail(TP_EHIP, "%s: cannot query the device", fn);
     tp::FusedArgs a = fused_args(geom, dstride, lpf_mask, s0, s1, s2, fat_zero, corr_stride,
-                                 corr_radius, c.tpi);
+                                 corr_radius, c.tpi, combined_stride, pair_mask);
     a.frames = (const tp::FrameDesc *)gpu_frames;
     a.nframes = nframes;
     a.work = gpu_work;
@@ -523,26 +529,31 @@ int tp_convert_correlate(const tp_geometry *geom, float **gpu_kernel_offsets, fl
                          float **gpu_images, const tp_task *gpu_tasks, float **gpu_clt,
                          size_t dstride, int num_tiles, int lpf_mask, int colors, float scale0,
                          float scale1, float scale2, float fat_zero, size_t corr_stride,
-                         int corr_radius, float *gpu_corrs, const tp_fused_config *config,
+                         int corr_radius, float *gpu_corrs, float *gpu_combined,
+                         size_t combined_stride, int pair_mask, const tp_fused_config *config,
                          tp_stream_t stream) {
     const char *fn = "tp_convert_correlate";
     int rc = fused_common(fn, geom, dstride, lpf_mask, colors, corr_stride, corr_radius);
     if (rc) return rc;
-    if (!gpu_kernel_offsets || !gpu_kernels || !gpu_images || !gpu_tasks || !gpu_corrs)
+    if (!gpu_kernel_offsets || !gpu_kernels || !gpu_images || !gpu_tasks)
         return fail(TP_EINVAL, "%s: NULL pointer argument", fn);
+    const size_t nn = 2 * corr_radius + 1;
+    if (gpu_combined && combined_stride < nn * nn)
+        return fail(TP_EINVAL, "%s: combined_stride < (2r+1)^2", fn);
     if (num_tiles < 0) return fail(TP_EINVAL, "%s: num_tiles < 0", fn);
     FusedCfg c;
     if ((rc = fused_config(fn, config, c))) return rc;
     g_err.clear();
     if (num_tiles == 0) return TP_OK;
     tp::FusedArgs a = fused_args(geom, dstride, lpf_mask, scale0, scale1, scale2, fat_zero,
-                                 corr_stride, corr_radius, c.tpi);
+                                 corr_stride, corr_radius, c.tpi, combined_stride, pair_mask);
     a.f0.tasks = gpu_tasks;
     a.f0.images = (const float *const *)gpu_images;
     a.f0.kernels = (const float *const *)gpu_kernels;
     a.f0.koffs = (const float *const *)gpu_kernel_offsets;
     a.f0.clt = gpu_clt;
     a.f0.corrs = gpu_corrs;
+    a.f0.combined = gpu_combined;
     a.f0.num_tiles = num_tiles;
     a.f0.item_start = 0;
     launch_fused<false>(a, c, colors, (num_tiles + c.tpi - 1) / c.tpi, as_stream(stream));
@@ -553,18 +564,19 @@ int tp_stream_convert_correlate(const tp_geometry *geom, tp_frame *gpu_frames, i
                                 int32_t *gpu_work, uint64_t *gpu_stamps, size_t dstride,
                                 int lpf_mask, int colors, float scale0, float scale1, float scale2,
                                 float fat_zero, size_t corr_stride, int corr_radius,
+                                size_t combined_stride, int pair_mask,
                                 const tp_fused_config *config, tp_stream_t stream) {
     g_err.clear();
     return enqueue_stream(geom, gpu_frames, nframes, gpu_work, gpu_stamps, dstride, lpf_mask, colors,
-                          scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, config,
-                          as_stream(stream), "tp_stream_convert_correlate");
+                          scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, combined_stride,
+                          pair_mask, config, as_stream(stream), "tp_stream_convert_correlate");
 }
 
 int tp_stream_graph_create(const tp_geometry *geom, tp_frame *gpu_frames, int nframes,
                            int32_t *gpu_work, uint64_t *gpu_stamps, size_t dstride, int lpf_mask,
                            int colors, float scale0, float scale1, float scale2, float fat_zero,
-                           size_t corr_stride, int corr_radius, const tp_fused_config *config,
-                           tp_graph_t *graph) {
+                           size_t corr_stride, int corr_radius, size_t combined_stride,
+                           int pair_mask, const tp_fused_config *config, tp_graph_t *graph) {
     const char *fn = "tp_stream_graph_create";
     if (!graph) return fail(TP_EINVAL, "%s: graph is NULL", fn);
     *graph = nullptr;
@@ -578,7 +590,8 @@ int tp_stream_graph_create(const tp_geometry *geom, tp_frame *gpu_frames, int nf
         return fail(TP_EHIP, "tp_stream_graph_create: begin capture: %s", hipGetErrorString(e));
     }
     int rc = enqueue_stream(geom, gpu_frames, nframes, gpu_work, gpu_stamps, dstride, lpf_mask, colors,
-                            scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, config, st, fn);
+                            scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, combined_stride,
+                            pair_mask, config, st, fn);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(st, &g);
     (void)hipStreamDestroy(st);

@@ -122,22 +122,26 @@ def _cfg(config):
 
 def convert_correlate(geom, gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt,
                       dstride, num_tiles, lpf_mask, colors, scale0, scale1, scale2, fat_zero,
-                      corr_stride, corr_radius, gpu_corrs, config=None, stream=None):
+                      corr_stride, corr_radius, gpu_corrs, gpu_combined=None, combined_stride=0,
+                      pair_mask=0x3F, config=None, stream=None):
     """Forward CLT + pair correlation of one frame in one pass (rows 6 task + pair);
-    gpu_clt may be None."""
-    _check_dev(gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt, gpu_corrs)
+    gpu_clt, gpu_corrs and gpu_combined (six-pair combination, row = task)
+    may each be None."""
+    _check_dev(gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tasks, gpu_clt, gpu_corrs,
+               gpu_combined)
     g = geom.c()
     _lib.call("tp_convert_correlate", ctypes.byref(g), _p(gpu_kernel_offsets), _p(gpu_kernels),
               _p(gpu_images), _p(gpu_tasks), _p(gpu_clt), int(dstride), int(num_tiles),
               int(lpf_mask), int(colors), float(scale0), float(scale1), float(scale2),
-              float(fat_zero), int(corr_stride), int(corr_radius), _p(gpu_corrs), _cfg(config),
+              float(fat_zero), int(corr_stride), int(corr_radius), _p(gpu_corrs),
+              _p(gpu_combined), int(combined_stride), int(pair_mask), _cfg(config),
               _stream(stream))
 
 
 # tp_frame (include/tp.h): one frame of the stream form
 FRAME_DTYPE = np.dtype([("tasks", "<u8"), ("images", "<u8"), ("kernels", "<u8"),
                         ("kernel_offsets", "<u8"), ("clt", "<u8"), ("corrs", "<u8"),
-                        ("num_tiles", "<i4"), ("item_start", "<i4")])
+                        ("combined", "<u8"), ("num_tiles", "<i4"), ("item_start", "<i4")])
 
 
 def wall_clock_hz():
@@ -149,7 +153,7 @@ class StreamRun:
     (tp_stream_convert_correlate) per call, or a captured graph of it."""
 
     def __init__(self, frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
-                 store_clt=False, config=None):
+                 store_clt=False, config=None, store_pairs=True, combined=False, pair_mask=0x3F):
         if not frames:
             raise ValueError("no frames")
         self.frames = frames
@@ -168,7 +172,9 @@ class StreamRun:
         for i, fb in enumerate(frames):
             desc[i] = (fb.tasks.data_ptr(), fb.p_images.data_ptr(), fb.p_kernels.data_ptr(),
                        fb.p_koffs.data_ptr(), fb.p_clt.data_ptr() if store_clt else 0,
-                       fb.corrs.data_ptr(), fb.num_tiles, 0)
+                       fb.corrs.data_ptr() if store_pairs else 0,
+                       fb.combined_buffer().data_ptr() if combined else 0, fb.num_tiles, 0)
+        self.pair_mask = pair_mask
         self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
         self.work = torch.zeros(2, dtype=torch.int32, device=dev)
         self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev)
@@ -180,7 +186,7 @@ class StreamRun:
         return (ctypes.byref(self.geom.c()), _p(self.desc), len(self.frames), _p(self.work),
                 _p(self.stamps), int(f0.dstride), int(self.lpf_mask), int(colors), float(s0),
                 float(s1), float(s2), float(fz), int(f0.corr_stride), int(f0.corr_radius),
-                ctypes.byref(self.config))
+                int(f0.combined_stride), int(self.pair_mask), ctypes.byref(self.config))
 
     def launch(self, stream=None):
         _lib.call("tp_stream_convert_correlate", *self._common(), _stream(stream))
@@ -283,16 +289,33 @@ class FrameBuffers:
                     self.num_corr, self.corr_idx, self.corr_stride, self.corr_radius, self.corrs,
                     stream)
 
+    def combined_buffer(self):
+        """[tiles][(2r+1)^2] six-pair combination rows (row = task), made on first use."""
+        n = 2 * self.corr_radius + 1
+        if getattr(self, "combined", None) is None or self.combined.shape[0] < max(1, self.num_tiles):
+            self.combined = torch.zeros((max(1, self.num_tiles), n * n), dtype=torch.float32,
+                                        device=self.device)
+        return self.combined
+
+    @property
+    def combined_stride(self):
+        n = 2 * self.corr_radius + 1
+        return n * n
+
     def fused(self, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
-              store_clt=False, config=None, stream=None):
+              store_clt=False, config=None, stream=None, store_pairs=True, combined=False,
+              pair_mask=0x3F):
         """Forward CLT + correlation in one pass; rows of self.corrs are
-        6 task + pair (self.corrs needs 6 num_tiles rows)."""
-        if self.corrs.shape[0] < 6 * self.num_tiles:
+        6 task + pair (self.corrs needs 6 num_tiles rows); with combined=True
+        also the six-pair combination into self.combined (row = task)."""
+        if store_pairs and self.corrs.shape[0] < 6 * self.num_tiles:
             raise ValueError("corrs needs 6 rows per task for the fused form")
+        comb = self.combined_buffer() if combined else None
         convert_correlate(self.geom, self.p_koffs, self.p_kernels, self.p_images, self.tasks,
                           self.p_clt if store_clt else None, self.dstride, self.num_tiles,
                           lpf_mask, colors, scales[0], scales[1], scales[2], fat_zero,
-                          self.corr_stride, self.corr_radius, self.corrs, config, stream)
+                          self.corr_stride, self.corr_radius, self.corrs if store_pairs else None,
+                          comb, self.combined_stride, pair_mask, config, stream)
 
 
 # texture parameters of the reference harness (src/dtt8x8.cu:791-814)
