@@ -273,6 +273,23 @@ def measure_stages(cfg, fb, idx, reps):
                          kw["min_agree"], kw["weight0"], kw["weight1"], kw["weight2"],
                          kw["dust_remove"], 0, stride, rbga)
     out["texture_rgba_ms"] = round(median_ms(texture, max(3, reps // 2)), 4)
+
+    # the whole per-frame pipeline (forward CLT, 6-pair correlation, 15 x 15
+    # combination, texture output), staged kernels against the fused kernel
+    # (CLT still stored for the texture step)
+    def staged_pipeline():
+        fb.convert()
+        fb.correlate(3, SCALES, FAT_ZERO)
+        tp.corr_combine(g, fb.num_corr, fb.corr_idx, fb.corrs, fb.corr_stride, RADIUS, 0x3F, comb, 225)
+        texture()
+
+    def fused_pipeline():
+        fb.fused(3, SCALES, FAT_ZERO, store_clt=True, combined=True)
+        texture()
+    out["pipeline_staged_ms"] = round(median_ms(staged_pipeline, max(3, reps // 2)), 4)
+    out["pipeline_fused_ms"] = round(median_ms(fused_pipeline, max(3, reps // 2)), 4)
+    out["fused_combined_only_ms"] = round(median_ms(
+        lambda: fb.fused(3, SCALES, FAT_ZERO, store_pairs=False, combined=True), reps), 4)
     # network on the full frame
     if os.path.exists(nn.DEFAULT_WEIGHTS):
         model = nn.DisparityCNN.load(nn.DEFAULT_WEIGHTS)

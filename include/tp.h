@@ -383,10 +383,15 @@ int tp_convert_correlate(const tp_geometry *geom,
                          const tp_fused_config *config,
                          tp_stream_t stream);
 
+#define TP_STREAM_WORK_INTS 288
+
 /* Forward CLT + correlation of a stream of frames (each with its own task
  * list, images, kernels and outputs) in one persistent launch: a fixed grid
- * of blocks_per_cu workgroups per CU takes work items from a global counter.
- * gpu_work: device int32[2] (queue counter, item count), reset by the call.
+ * of blocks_per_cu workgroups per CU takes work items from 8 per-XCD queues
+ * (contiguous eighths of the items, with stealing).
+ * gpu_work: device int32[TP_STREAM_WORK_INTS], reset by the call: the 8 queue
+ * counters at [32 x] (one 128-byte line each, so that the claims of
+ * different XCDs do not contend for one line) and the item count at [256].
  * gpu_stamps: NULL or device uint64[nframes][2], set to each frame's first
  * item start and last item end (GPU wall clock, wall_clock_hz ticks/s). */
 int tp_stream_convert_correlate(const tp_geometry *geom,

@@ -158,7 +158,7 @@ def test_stream_matches_single_frames(stream, cfg):
         assert rel_err(got, stream["refs"][i].cpu().numpy()) <= TOL_STAGED, (cfg, i)
     times = run.frame_times_s()
     assert (times[:, 1] >= times[:, 0]).all() and times[:, 0].min() == 0.0
-    assert int(run.work[0].item()) >= int(run.work[1].item())  # queue drained
+    assert run.drained()
 
 
 def test_stream_graph_replays(stream):

@@ -190,7 +190,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
                                              int ustride = 72) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
-    const tp_task *tk = a.tasks + task_num;
+    gtask *tk = gtasks(a.tasks) + task_num;
     const float cx = tk->xy[cam][0], cy = tk->xy[cam][1];
 
     // (1) kernel cell, :3129-3134
@@ -201,10 +201,10 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     const float kdx = (cx - (float)(ktx << 4)) + 8.0f;
     const float kdy = (cy - (float)(kty << 4)) + 8.0f;
     const size_t kfi = (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + COLOR;
-    gfloat *ex = gptr(a.koffs[cam] + kfi * 8);
+    gfloat *ex = gptr(gelem(a.koffs, cam) + kfi * 8);
     const float offx = fmaf(ex[5], kdy, fmaf(ex[4], kdx, ex[0]));
     const float offy = fmaf(ex[7], kdy, fmaf(ex[6], kdx, ex[1]));
-    const float *kern = a.kernels[cam] + kfi * 256;  // read through gptr4
+    const float *kern = gelem(a.kernels, cam) + kfi * 256;  // read through gptr4
 
     // (2) integer top-left, residual shift, :3182-3247
     const float px = (cx - 8.0f) - offx;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // Pixel offsets (32-bit).  Interior windows (the usual case) are linear
     // in the row; windows reaching past the right/bottom edge read the
     // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
-    gfloat *img = gptr(a.images[cam]);
+    gfloat *img = gptr(gelem(a.images, cam));
     // pixel loads; with TP_IMG_NT they are non-temporal (evict-first in L2),
     // leaving the aberration kernels, which 4 tiles share, resident longer
     auto pix = [&](int o) -> float {
@@ -266,23 +266,40 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
         const int pr = (itly & 1) ^ color0;
         const int xo = pc ? x1 : x0;
         const float wc = pc ? whc1 : whc0;
+        // all loads are issued before the first use (the scheduling barrier
+        // keeps the compiler from pairing each load with its use, which
+        // serialises the HBM latencies when it is short of registers)
+        float px[8];
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
-            const bool use_a = (ra & 1) == pr;
-            const int r = use_a ? ra : rb;
-            const float w = use_a ? wv[ra] : wv[rb];
-            f[l] = pix(rowoff(r) + xo) * w * wc;
+            px[l] = pix(rowoff((ra & 1) == pr ? ra : rb) + xo);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            const float w = (ra & 1) == pr ? wv[ra] : wv[rb];
+            f[l] = px[l] * w * wc;
         }
     } else {
         // G sites: (row + col + itlx + itly) even; both source rows used
         const int S = (itlx + itly) & 1;
+        float pa[8], pb[8];
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
             const int pca = (ra + S) & 1;  // column parity for row ra; row rb takes the other
-            const float da = pix(rowoff(ra) + (pca ? x1 : x0)) * wv[ra];
-            const float db = pix(rowoff(rb) + (pca ? x0 : x1)) * wv[rb];
+            pa[l] = pix(rowoff(ra) + (pca ? x1 : x0));
+            pb[l] = pix(rowoff(rb) + (pca ? x0 : x1));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            const int pca = (ra + S) & 1;
+            const float da = pa[l] * wv[ra];
+            const float db = pb[l] * wv[rb];
             f[l] = fmaf(da, pca ? whc1 : whc0, db * (pca ? whc0 : whc1));
             fs[l] = fmaf(da, pca ? whs1 : whs0, db * (pca ? whs0 : whs1));
         }
@@ -459,9 +476,9 @@ __device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const L
     float q0[8], q1[8], q2[8], q3[8];
     convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, {}, 256);
     // (12) store [q][l][k], :3600-3630; out-of-grid tiles are skipped
-    const int tile = task_tile(a.g, a.tasks[task_num].txy);
+    const int tile = task_tile(a.g, gtasks(a.tasks)[task_num].txy);
     const bool st = valid && tile >= 0;
-    float *dst = a.clt[cam] + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
+    float *dst = gelem(a.clt, cam) + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
 #if TP_CONVERT_LDS_STORE
     // The unit's 1 KB block goes through LDS so that store j writes float4
     // j * 8 + li: every store instruction covers whole 128-byte lines of
@@ -515,7 +532,7 @@ void k_convert_correct_tiles(ConvertArgs a) {
         } else {
             task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
         }
-        valid = valid && (a.tasks[task_num].task != 0);
+        valid = valid && (gtasks(a.tasks)[task_num].task != 0);
         if (wave == 0)
             convert_store_unit<0>(a, lf, slice, li, cam, task_num, valid);
         else if (wave == 1)

@@ -20,8 +20,9 @@
 // Work items are runs of tiles_per_item consecutive tiles (in the visiting
 // order of swizzle_strip) of one frame.  Single-frame launches map one
 // workgroup to one item.  Stream launches (QUEUE) are persistent: a fixed
-// grid of workgroups takes items from a global counter until every frame's
-// items are done; frames are located by binary search on item_start.
+// grid of workgroups takes items from 8 per-XCD queues (contiguous eighths
+// of all items, with stealing) until every frame's items are done; frames
+// are located by binary search on item_start.
 #pragma once
 
 #include "k_convert.hpp"
@@ -29,6 +30,10 @@
 #include "k_combine.hpp"
 
 namespace tp {
+
+constexpr int TP_WORK_LINE = 32;    // ints per 128-byte line: one claim counter per line
+constexpr int TP_WORK_ITEMS = 256;  // item count, after the 8 counter lines
+static_assert(TP_WORK_ITEMS < TP_STREAM_WORK_INTS, "stream work buffer");
 
 // Binary-identical to tp_frame (include/tp.h).
 struct FrameDesc {
@@ -48,7 +53,7 @@ struct FusedArgs {
     FrameDesc f0;              // single-frame launch
     const FrameDesc *frames;   // stream launch: device array [nframes]
     int nframes;
-    int *work;                 // stream launch: [0] next item, [1] number of items
+    int *work;                 // stream launch: per-XCD claims at [TP_WORK_LINE x], items at [TP_WORK_ITEMS]
     unsigned long long *stamps;  // stream launch: per frame {first start, last end} or null
     int tiles_per_item;        // multiple of GT
     size_t dstride;
@@ -61,8 +66,18 @@ struct FusedArgs {
     int pair_mask;            // pairs summed into the combined tile
 };
 
+// Occupancy: the FP32-DCT kernel with two tiles per group fits 168 VGPRs
+// without spilling, i.e. 3 waves per SIMD (4 workgroups per CU); left to
+// itself the compiler takes a few more registers in the queue form and
+// drops to 2 waves per SIMD (2 workgroups per CU), 1.6x slower.  The other
+// forms need more than 168 registers and keep the compiler's choice.
+template <int GT, bool QUEUE, int COLORS, int DCT>
+constexpr int fused_min_waves() {
+    return (GT == 2 && DCT == 0 && QUEUE) ? TP_FUSED_QWAVES : TP_FUSED_WAVES;
+}
+
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_FUSED_WAVES)))
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(fused_min_waves<GT, QUEUE, COLORS, DCT>())))
 void k_fused(FusedArgs a) {
     static_assert(GT == 2 || GT == 4, "tiles per group");
     constexpr int NSLOT = 6 * GT;
@@ -81,27 +96,59 @@ void k_fused(FusedArgs a) {
     corr_out_offsets(lane, a.radius, src_off);
     const float fz2 = a.fat_zero * a.fat_zero;
     const Geo &g = a.g;
-    const int total = QUEUE ? a.work[1] : (int)gridDim.x;
+    const int total = QUEUE ? a.work[TP_WORK_ITEMS] : (int)gridDim.x;
     int item = QUEUE ? 0 : xcd_remap(blockIdx.x, gridDim.x);
 
     while (true) {
         if (QUEUE) {
-            if (threadIdx.x == 0) s_item = atomicAdd(&a.work[0], 1);
+            // One queue per XCD over a contiguous eighth of the items, so
+            // that neighbouring tiles (which share kernel cells and image
+            // rows) are converted under one L2; a workgroup (on XCD
+            // blockIdx.x % 8) drains its own queue first, then the others.
+            if (threadIdx.x == 0) {
+                int it = -1;
+                const int x0 = blockIdx.x & 7;
+                for (int k = 0; k < 8 && it < 0; k++) {
+                    const int x = (x0 + k) & 7;
+                    const int lo = (int)((long long)total * x / 8);
+                    const int hi = (int)((long long)total * (x + 1) / 8);
+                    const int c = atomicAdd(&a.work[x * TP_WORK_LINE], 1);
+                    if (c < hi - lo) it = lo + c;
+                }
+                s_item = it;
+            }
             __syncthreads();
-            item = s_item;
+            // the same value in every lane: keep it (and the frame descriptor
+            // and pointers derived from it) in scalar registers
+            item = __builtin_amdgcn_readfirstlane(s_item);
             __syncthreads();  // s_item is rewritten by the next iteration
-            if (item >= total) break;
+            if (item < 0) break;  // every queue is drained
         }
         int f = 0;
+        // frame descriptors live in global memory (scalar loads, not flat ones)
+        typedef __attribute__((address_space(1))) const FrameDesc gframe;
+        gframe *frames = (gframe *)a.frames;
         if (QUEUE) {  // last frame with item_start <= item
             int lo = 0, hi = a.nframes - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (a.frames[mid].item_start <= item) lo = mid; else hi = mid - 1;
+                if (frames[mid].item_start <= item) lo = mid; else hi = mid - 1;
             }
             f = lo;
         }
-        const FrameDesc fr = QUEUE ? a.frames[f] : a.f0;
+        FrameDesc fr = a.f0;
+        if (QUEUE) {
+            gframe &d = frames[f];
+            fr.tasks = d.tasks;
+            fr.images = d.images;
+            fr.kernels = d.kernels;
+            fr.koffs = d.koffs;
+            fr.clt = d.clt;
+            fr.corrs = d.corrs;
+            fr.combined = d.combined;
+            fr.num_tiles = d.num_tiles;
+            fr.item_start = d.item_start;
+        }
         if (QUEUE && a.stamps && threadIdx.x == 0)
             atomicMin(&a.stamps[2 * f], (unsigned long long)wall_clock64());
         ConvertArgs ca;
@@ -126,7 +173,7 @@ void k_fused(FusedArgs a) {
                 const int s = gs + tl;
                 bool valid = s < s1;
                 const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
-                valid = valid && fr.tasks[task_num].task != 0;
+                valid = valid && gtasks(fr.tasks)[task_num].task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
                 const int color = wave;
                 float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * 256;  // units 768 floats apart
@@ -144,9 +191,9 @@ void k_fused(FusedArgs a) {
                     *(float4 *)(d + qq * 64 + 4) = make_float4(q[4], q[5], q[6], q[7]);
                 }
                 if (fr.clt) {
-                    const int tile = task_tile(g, fr.tasks[task_num].txy);
+                    const int tile = task_tile(g, gtasks(fr.tasks)[task_num].txy);
                     if (valid && tile >= 0)
-                        store_clt_rows(fr.clt[cam] + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
+                        store_clt_rows(gelem(fr.clt, cam) + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
                                        q0, q1, q2, q3);
                 }
             }
@@ -163,7 +210,7 @@ void k_fused(FusedArgs a) {
                 const int s = gs + tl;
                 if (slot < NSLOT && s < s1) {
                     const int task_num = swizzle_strip(s, g.TX, fr.num_tiles);
-                    const tp_task *tk = fr.tasks + task_num;
+                    gtask *tk = gtasks(fr.tasks) + task_num;
                     if (((tk->task >> (TP_TASK_CORR_BITS + pair)) & 1) && task_tile(g, tk->txy) >= 0)
                         row = 6 * task_num + pair;
                 }
@@ -203,7 +250,7 @@ void k_fused(FusedArgs a) {
                     const int s = gs + tl;
                     if (s >= s1) continue;
                     const int task_num = swizzle_strip(s, g.TX, fr.num_tiles);
-                    const tp_task *tk = fr.tasks + task_num;
+                    gtask *tk = gtasks(fr.tasks) + task_num;
                     if (tk->task == 0 || task_tile(g, tk->txy) < 0) continue;
                     const int cy = oo / n - r, cx = oo % n - r;
                     float sum = 0.f;
@@ -235,8 +282,8 @@ __global__ void k_stream_setup(FrameDesc *frames, int nframes, int tiles_per_ite
             frames[f].item_start = acc;
             acc += (max(frames[f].num_tiles, 0) + tiles_per_item - 1) / tiles_per_item;
         }
-        work[0] = 0;
-        work[1] = acc;
+        for (int x = 0; x < 8; x++) work[x * TP_WORK_LINE] = 0;  // per-XCD claim counters
+        work[TP_WORK_ITEMS] = acc;                              // number of items
     }
     if (stamps)
         for (int i = threadIdx.x; i < nframes; i += blockDim.x) {

@@ -19,6 +19,8 @@
 // whole generate_RBGA sequence is stream-ordered and graph-capturable.
 #pragma once
 
+#include <climits>
+
 #include "tp_common.hpp"
 #include "k_imclt.hpp"
 
@@ -67,19 +69,24 @@ __device__ __forceinline__ float alpha_fade(int m, int i, int j) {
     }
 }
 
-template <int COLORS>
-__global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
-    __shared__ float M[12 * 256];  // [cam * colors + color][256]
-    __shared__ __align__(16) union U {
+// LDS of one texture tile
+struct TexShared {
+    float M[12 * 256];  // [cam * colors + color][256]
+    union {
         float scratch[4][8][4][72];  // phase 1: imclt transposes
         struct {
             float tmp[12 * 256];
             float D[12 * 256];
         } p2;
     } u;
-    const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
-    const int e = blockIdx.x;
-    if (e >= n) return;
+};
+
+// One texture list entry e (of n) by the whole workgroup; returns early
+// (uniformly) for entries without a texture bit or off the grid.
+template <int COLORS>
+__device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, TexShared &sh) {
+    float *M = sh.M;
+    auto &u = sh.u;
     const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
     const int code = list[e];
     if (!(code & (1 << TP_LIST_TEXTURE_BIT))) return;
@@ -259,6 +266,18 @@ __global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
     for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
 }
 
+// A fixed grid of workgroups loops over the list (its length may only be
+// known on the device), one entry at a time.
+template <int COLORS>
+__global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
+    __shared__ __align__(16) TexShared sh;
+    const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        texture_tile<COLORS>(a, e, n, sh);
+        __syncthreads();  // the LDS of this entry is rewritten by the next one
+    }
+}
+
 // ---------------- texture list (generate_RBGA / prepare_texture_list) ----------------
 
 // zero the tile occupancy map (kept in the first tilesx*height entries of
@@ -287,16 +306,51 @@ __global__ void k_tex_mark(const tp_task *tasks, int n, int *indices, int TX, in
     indices[x + y * TX] = 1;
 }
 
-__global__ void k_tex_neighbors(tp_task *tasks, int n, const int *indices, int *woi, int TX, int TY) {
+// min / max over the wavefront's active lanes (inactive lanes pass the
+// identity), then one atomic per wavefront
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Launched with TP_NEIGHBOR_THREADS threads per block: the window of
+// interest is reduced per wavefront, then per block through LDS, so one
+// block makes one set of 4 same-address atomics (these serialise in L2).
+#define TP_NEIGHBOR_THREADS 1024
+__global__ __launch_bounds__(TP_NEIGHBOR_THREADS) void k_tex_neighbors(tp_task *tasks, int n,
+                                                                       const int *indices, int *woi,
+                                                                       int TX, int TY) {
+    __shared__ int red[4][TP_NEIGHBOR_THREADS / 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int task = tasks[t].task;
-    int x, y;
-    if (!(task & TP_TASK_TEXTURE_BITS) || !task_xy(tasks[t], TX, TY, x, y)) return;
-    atomicMin(woi + 0, x);
-    atomicMin(woi + 1, y);
-    atomicMax(woi + 2, x);
-    atomicMax(woi + 3, y);
+    const int task = t < n ? tasks[t].task : 0;
+    int x = 0, y = 0;
+    const bool act = t < n && (task & TP_TASK_TEXTURE_BITS) && task_xy(tasks[t], TX, TY, x, y);
+    const int x0 = wave_min(act ? x : INT_MAX), y0 = wave_min(act ? y : INT_MAX);
+    const int x1 = wave_max(act ? x : INT_MIN), y1 = wave_max(act ? y : INT_MIN);
+    const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (__lane_id() == 0) {
+        red[0][wv] = x0;
+        red[1][wv] = y0;
+        red[2][wv] = x1;
+        red[3][wv] = y1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        const int bx0 = wave_min(l < nw ? red[0][l] : INT_MAX), by0 = wave_min(l < nw ? red[1][l] : INT_MAX);
+        const int bx1 = wave_max(l < nw ? red[2][l] : INT_MIN), by1 = wave_max(l < nw ? red[3][l] : INT_MIN);
+        if (l == 0 && bx0 != INT_MAX) {
+            atomicMin(woi + 0, bx0);
+            atomicMin(woi + 1, by0);
+            atomicMax(woi + 2, bx1);
+            atomicMax(woi + 3, by1);
+        }
+    }
+    if (!act) return;
     int d = 0;
     if (y > 0 && indices[x + (y - 1) * TX]) d |= 1;
     if (x < TX - 1 && indices[x + 1 + y * TX]) d |= 2;
@@ -305,21 +359,50 @@ __global__ void k_tex_neighbors(tp_task *tasks, int n, const int *indices, int *
     tasks[t].task = ((task ^ d) & TP_TASK_TEXTURE_BITS) ^ task;
 }
 
-__global__ void k_tex_gen_list(const tp_task *tasks, int n, int *indices, int *num, const int *woi,
-                               int TX, int TY, int TYA) {
+// Launched with TP_NEIGHBOR_THREADS threads per block.  Positions in the
+// 8 sub-lists: lane rank within the wavefront (ballot), wavefront offset
+// within the block (LDS prefix), block offset from one atomic per
+// (block, sub-list); tiles of a block take consecutive positions in thread
+// order.
+__global__ __launch_bounds__(TP_NEIGHBOR_THREADS) void k_tex_gen_list(const tp_task *tasks, int n,
+                                                                      int *indices, int *num,
+                                                                      const int *woi, int TX, int TY,
+                                                                      int TYA) {
+    __shared__ int wcount[TP_NEIGHBOR_THREADS / 64][8];
+    __shared__ int bbase[8];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int task = tasks[t].task & TP_TASK_TEXTURE_BITS;
-    int x, y;
-    if (!task || !task_xy(tasks[t], TX, TY, x, y)) return;
+    const int task = t < n ? (tasks[t].task & TP_TASK_TEXTURE_BITS) : 0;
+    int x = 0, y = 0;
+    const bool act = task && task_xy(tasks[t], TX, TY, x, y);
     const int quarter = TX * (TYA >> 2);
-    const bool border = (x == woi[0]) || (y == woi[1]) || (x == TX - 1) || (y == woi[3]);
+    const bool border = act && ((x == woi[0]) || (y == woi[1]) || (x == TX - 1) || (y == woi[3]));
     int head = 0, slot = 0;
     if (x & 1) { head += quarter; slot += 2; }
     if (y & 1) { head += 2 * quarter; slot += 4; }
     if (border) { head += quarter - 1; slot += 1; }
-    const int pos = atomicAdd(num + slot, 1);
-    if (pos >= quarter) return;  // only possible with duplicate tiles in the task list
+    const int lane = __lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int rank = 0;
+#pragma unroll
+    for (int sl = 0; sl < 8; sl++) {
+        const unsigned long long m = __ballot(act && slot == sl);
+        if (act && slot == sl) rank = __popcll(m & ((1ull << lane) - 1));
+        if (lane == 0) wcount[wv][sl] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {  // exclusive prefix over wavefronts, one atomic per sub-list
+        const int sl = threadIdx.x;
+        int acc = 0;
+        for (int w = 0; w < nw; w++) {
+            const int c = wcount[w][sl];
+            wcount[w][sl] = acc;
+            acc += c;
+        }
+        bbase[sl] = acc ? atomicAdd(num + sl, acc) : 0;
+    }
+    __syncthreads();
+    if (!act) return;
+    const int pos = bbase[slot] + wcount[wv][slot] + rank;
+    if (pos >= quarter) return;  // only with duplicate tiles in the task list
     indices[head + (border ? -pos : pos)] = task | ((x + y * TX) << TP_CORR_NTILE_SHIFT) | (1 << TP_LIST_TEXTURE_BIT);
 }
 

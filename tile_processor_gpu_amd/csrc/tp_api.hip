@@ -291,6 +291,10 @@ namespace {
 
 int launch_textures(const tp::TexArgs &a, int grid, hipStream_t st) {
     if (grid <= 0) return TP_OK;
+    // workgroups loop over the list: at most 3 resident per CU (LDS), a few
+    // rounds of them
+    const int cus = num_cus();
+    grid = std::min(grid, (cus > 0 ? cus : 256) * 3 * 4);
     if (a.colors == 3)
         hipLaunchKernelGGL(tp::k_textures_accumulate<3>, dim3(grid), dim3(256), 0, st, a);
     else
@@ -350,11 +354,14 @@ int enqueue_texture_list(const tp_geometry *geom, tp_task *tasks, int num_tiles,
         hipLaunchKernelGGL(tp::k_tex_mark, dim3(grid), dim3(256), 0, st, tasks, num_tiles, indices,
                            g.TX, g.TY);
         if ((rc = check_launch("mark_texture_tiles"))) return rc;
-        hipLaunchKernelGGL(tp::k_tex_neighbors, dim3(grid), dim3(256), 0, st, tasks, num_tiles,
-                           indices, woi, g.TX, g.TY);
+        hipLaunchKernelGGL(tp::k_tex_neighbors,
+                           dim3((num_tiles + TP_NEIGHBOR_THREADS - 1) / TP_NEIGHBOR_THREADS),
+                           dim3(TP_NEIGHBOR_THREADS), 0, st, tasks, num_tiles, indices, woi, g.TX, g.TY);
         if ((rc = check_launch("mark_texture_neighbor_tiles"))) return rc;
-        hipLaunchKernelGGL(tp::k_tex_gen_list, dim3(grid), dim3(256), 0, st, tasks, num_tiles,
-                           indices, num, woi, g.TX, g.TY, g.TYA);
+        hipLaunchKernelGGL(tp::k_tex_gen_list,
+                           dim3((num_tiles + TP_NEIGHBOR_THREADS - 1) / TP_NEIGHBOR_THREADS),
+                           dim3(TP_NEIGHBOR_THREADS), 0, st, tasks, num_tiles, indices, num, woi, g.TX,
+                           g.TY, g.TYA);
         if ((rc = check_launch("gen_texture_list"))) return rc;
     }
     hipLaunchKernelGGL(tp::k_tex_finish_woi, dim3(1), dim3(64), 0, st, woi);

@@ -28,6 +28,9 @@
 #ifndef TP_FUSED_WAVES
 #define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
 #endif
+#ifndef TP_FUSED_QWAVES
+#define TP_FUSED_QWAVES 3  // queue form of the FP32 fused kernel (two tiles per group)
+#endif
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
@@ -196,6 +199,13 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const float gfloat;
 typedef __attribute__((address_space(1))) const f32x4 gf32x4;
 __device__ __forceinline__ gfloat *gptr(const float *p) { return (gfloat *)p; }
+typedef __attribute__((address_space(1))) const tp_task gtask;
+__device__ __forceinline__ gtask *gtasks(const tp_task *p) { return (gtask *)p; }
+// element i of a per-camera pointer array held in global memory
+template <class T>
+__device__ __forceinline__ T gelem(const T *arr, int i) {
+    return ((__attribute__((address_space(1))) const T *)arr)[i];
+}
 __device__ __forceinline__ gf32x4 *gptr4(const float *p) { return (gf32x4 *)p; }
 
 // 16-byte store of an output that is not read again by this kernel: with

@@ -139,6 +139,7 @@ def convert_correlate(geom, gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tas
 
 
 # tp_frame (include/tp.h): one frame of the stream form
+STREAM_WORK_INTS = 288  # TP_STREAM_WORK_INTS
 FRAME_DTYPE = np.dtype([("tasks", "<u8"), ("images", "<u8"), ("kernels", "<u8"),
                         ("kernel_offsets", "<u8"), ("clt", "<u8"), ("corrs", "<u8"),
                         ("combined", "<u8"), ("num_tiles", "<i4"), ("item_start", "<i4")])
@@ -152,8 +153,16 @@ class StreamRun:
     """The stream form over a list of FrameBuffers: one persistent launch
     (tp_stream_convert_correlate) per call, or a captured graph of it."""
 
+    def drained(self):
+        """True when every per-XCD queue has handed out all its items (call
+        after synchronising)."""
+        w = self.work.cpu().numpy().astype(np.int64)
+        total = int(w[256])
+        return all(w[32 * x] >= total * (x + 1) // 8 - total * x // 8 for x in range(8))
+
     def __init__(self, frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
-                 store_clt=False, config=None, store_pairs=True, combined=False, pair_mask=0x3F):
+                 store_clt=False, config=None, store_pairs=True, combined=False, pair_mask=0x3F,
+                 stamps=True):
         if not frames:
             raise ValueError("no frames")
         self.frames = frames
@@ -176,8 +185,10 @@ class StreamRun:
                        fb.combined_buffer().data_ptr() if combined else 0, fb.num_tiles, 0)
         self.pair_mask = pair_mask
         self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
-        self.work = torch.zeros(2, dtype=torch.int32, device=dev)
-        self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev)
+        # per-XCD queue counters (one 128-byte line each) and the item count
+        self.work = torch.zeros(STREAM_WORK_INTS, dtype=torch.int32, device=dev)
+        # per-frame GPU time stamps (two atomics per work item on the frame's words)
+        self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev) if stamps else None
         self.graph = None
 
     def _common(self):
@@ -216,6 +227,8 @@ class StreamRun:
     def frame_times_s(self):
         """Per frame (start, end) of its work items in seconds after the
         first start, from the GPU wall clock (call after synchronising)."""
+        if self.stamps is None:
+            raise ValueError("this StreamRun keeps no time stamps")
         raw = self.stamps.cpu().numpy()
         hz = wall_clock_hz()
         done = raw[:, 0] != -1  # frames without work items keep the initial stamps
