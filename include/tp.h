@@ -332,8 +332,8 @@ int tp_corr_disparity(const float *gpu_tiles,
 
 /* Launch configuration (NULL: defaults). */
 typedef struct tp_fused_config {
-    int32_t tiles_per_item;   /* tiles per work item, a multiple of tiles_per_group (default 8) */
-    int32_t tiles_per_group;  /* tiles held in LDS at once: 2 or 4 (default 2) */
+    int32_t tiles_per_item;   /* tiles per work item, a multiple of tiles_per_group (default 12) */
+    int32_t tiles_per_group;  /* tiles held in LDS at once: 2 or 4 (default 4) */
     int32_t blocks_per_cu;    /* stream form: resident workgroups per CU, 1..8 (default 4) */
     int32_t dct;              /* forward DCT-IV/DST-IV: 0 FP32 butterflies (default) */
 } tp_fused_config;

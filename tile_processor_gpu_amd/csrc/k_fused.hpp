@@ -66,14 +66,14 @@ struct FusedArgs {
     int pair_mask;            // pairs summed into the combined tile
 };
 
-// Occupancy: the FP32-DCT kernel with two tiles per group fits 168 VGPRs
-// without spilling, i.e. 3 waves per SIMD (4 workgroups per CU); left to
-// itself the compiler takes a few more registers in the queue form and
-// drops to 2 waves per SIMD (2 workgroups per CU), 1.6x slower.  The other
-// forms need more than 168 registers and keep the compiler's choice.
+// Occupancy: the FP32-DCT kernel fits 168 VGPRs, i.e. 3 waves per SIMD;
+// left to itself the compiler takes a few more registers in the queue form
+// and drops to 2 waves per SIMD, 1.2-1.6x slower, so the queue form asks
+// for 3.  The matrix-core forms need more than 168 registers and keep the
+// compiler's choice.
 template <int GT, bool QUEUE, int COLORS, int DCT>
 constexpr int fused_min_waves() {
-    return (GT == 2 && DCT == 0 && QUEUE) ? TP_FUSED_QWAVES : TP_FUSED_WAVES;
+    return (DCT == 0 && QUEUE) ? TP_FUSED_QWAVES : TP_FUSED_WAVES;
 }
 
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>
@@ -167,7 +167,9 @@ void k_fused(FusedArgs a) {
 
         for (int gs = s0; gs < s1; gs += GT) {
             // (1) forward CLT of the group into LDS
-#pragma unroll
+            // one pass converts 2 tiles (8 units per wavefront); kept as a
+            // loop so that two passes' registers are not live together
+#pragma unroll 1
             for (int pass = 0; pass < GT / 2; pass++) {
                 const int tl = pass * 2 + (unit >> 2);
                 const int s = gs + tl;

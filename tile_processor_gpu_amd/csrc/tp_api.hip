@@ -62,7 +62,7 @@ struct FusedCfg {
 };
 
 int fused_config(const char *fn, const tp_fused_config *c, FusedCfg &o) {
-    o = {8, 2, 4, 0};
+    o = {12, 4, 4, 0};
     if (c) o = {c->tiles_per_item, c->tiles_per_group, c->blocks_per_cu, c->dct};
     if (o.gt != 2 && o.gt != 4) return fail(TP_EINVAL, "%s: tiles_per_group must be 2 or 4", fn);
     if (o.tpi < o.gt || o.tpi % o.gt || o.tpi > 4096)

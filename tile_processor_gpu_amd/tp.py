@@ -105,7 +105,7 @@ class FusedConfig(ctypes.Structure):
     _fields_ = [("tiles_per_item", ctypes.c_int32), ("tiles_per_group", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("dct", ctypes.c_int32)]
 
-    def __init__(self, tiles_per_item=8, tiles_per_group=2, blocks_per_cu=4, dct=0):
+    def __init__(self, tiles_per_item=12, tiles_per_group=4, blocks_per_cu=4, dct=0):
         super().__init__(tiles_per_item, tiles_per_group, blocks_per_cu, dct)
 
     def as_dict(self):
