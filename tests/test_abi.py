@@ -82,3 +82,17 @@ def test_imclt_and_lpf_reject_bad_arguments():
 def test_python_wrapper_raises_on_error():
     with pytest.raises(_lib.TPError):
         _lib.call("tp_set_lpf", 9, NZ)
+
+
+def test_hbm_probe_library():
+    """The benchmark's bandwidth probe loads and rejects bad arguments
+    without launching anything."""
+    from tile_processor_gpu_amd import build as tpb
+    P = ctypes.CDLL(tpb.PROBE_LIB)
+    P.tp_probe_hbm.restype = ctypes.c_int
+    P.tp_probe_hbm.argtypes = [ctypes.c_int, VP, VP, VP, ctypes.c_size_t, ctypes.c_int, VP]
+    assert P.tp_probe_hbm(0, NZ, NZ, NZ, 0, 4, None) == -1        # no bytes
+    assert P.tp_probe_hbm(0, NZ, NZ, NZ, 17, 4, None) == -1       # not a multiple of 16
+    assert P.tp_probe_hbm(0, NZ, NZ, NZ, 1024, 0, None) == -1     # bad blocks per CU
+    assert P.tp_probe_hbm(0, None, NZ, NZ, 1024, 4, None) == -1   # read without a source
+    assert P.tp_probe_hbm(3, NZ, None, NZ, 1024, 4, None) == -1   # mix without a destination

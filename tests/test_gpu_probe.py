@@ -1,0 +1,35 @@
+"""The benchmark's HBM probe kernels move the bytes they claim to move."""
+import ctypes
+
+import pytest
+import torch
+
+from tile_processor_gpu_amd import build as tpb
+
+pytestmark = pytest.mark.gpu
+VP = ctypes.c_void_p
+
+
+def probe():
+    P = ctypes.CDLL(tpb.PROBE_LIB)
+    P.tp_probe_hbm.restype = ctypes.c_int
+    P.tp_probe_hbm.argtypes = [ctypes.c_int, VP, VP, VP, ctypes.c_size_t, ctypes.c_int, VP]
+    return P
+
+
+def test_copy_and_mix():
+    P = probe()
+    n = (1 << 20) + 4  # floats; not a multiple of the grid
+    g = torch.Generator(device="cuda:0").manual_seed(3)
+    src = torch.rand(n, device="cuda:0", generator=g)
+    dst = torch.zeros(3 * n, device="cuda:0")
+    sink = torch.zeros(1, device="cuda:0")
+    assert P.tp_probe_hbm(2, VP(src.data_ptr()), VP(dst.data_ptr()), VP(sink.data_ptr()), 4 * n, 4, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:n], src)
+    assert P.tp_probe_hbm(3, VP(src.data_ptr()), VP(dst.data_ptr()), VP(sink.data_ptr()), 4 * n, 4, None) == 0
+    assert P.tp_probe_hbm(0, VP(src.data_ptr()), None, VP(sink.data_ptr()), 4 * n, 8, None) == 0
+    assert P.tp_probe_hbm(1, None, VP(dst.data_ptr()), None, 4 * n, 8, None) == 0
+    torch.cuda.synchronize()
+    assert float(sink.item()) == 0.0
+    assert torch.equal(dst[n:2 * n], src * 2) and torch.equal(dst[2 * n:], src * 3)

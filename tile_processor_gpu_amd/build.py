@@ -25,12 +25,10 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=True, out=None, defines=()):
-    """Compile csrc/tp_api.hip (the whole library is one translation unit).
-    defines: extra -D tuning knobs (see csrc/tp_common.hpp); out: library
-    path (default: the in-tree libtp.so)."""
-    src = os.path.join(CSRC, "tp_api.hip")
-    out = out or LIB
+PROBE_LIB = os.path.join(HERE, "libtp_probe.so")
+
+
+def _compile(src, out, defines, verbose):
     tmp = out + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-o", tmp, src]
     if verbose:
@@ -40,11 +38,22 @@ def build(verbose=True, out=None, defines=()):
     return out
 
 
+def build(verbose=True, out=None, defines=(), probe=True):
+    """Compile csrc/tp_api.hip (the whole library is one translation unit)
+    and, with probe=True, the HBM bandwidth probe csrc/hbm_probe.hip that the
+    benchmark uses (libtp_probe.so).  defines: extra -D tuning knobs (see
+    csrc/tp_common.hpp); out: library path (default: the in-tree libtp.so)."""
+    out = _compile(os.path.join(CSRC, "tp_api.hip"), out or LIB, defines, verbose)
+    if probe:
+        _compile(os.path.join(CSRC, "hbm_probe.hip"), PROBE_LIB, (), verbose)
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("-o", "--out", default=None)
     ap.add_argument("-D", "--define", action="append", default=[])
     a = ap.parse_args()
-    build(out=a.out, defines=a.define)
+    build(out=a.out, defines=a.define, probe=a.out is None)
     sys.exit(0)
