@@ -41,7 +41,18 @@ struct ConvertArgs {
     int num_tiles;
     int lpf_mask;
     int pairs_per_block;
+    // this lane's camera: entries of the per-camera pointer arrays, read
+    // once per kernel (or work item) by resolve_cam rather than per unit
+    const float *l_koffs, *l_kern, *l_img;
+    float *l_clt;
 };
+
+__device__ __forceinline__ void resolve_cam(ConvertArgs &a, int cam) {
+    a.l_koffs = gelem(a.koffs, cam);
+    a.l_kern = gelem(a.kernels, cam);
+    a.l_img = gelem(a.images, cam);
+    a.l_clt = a.clt ? gelem(a.clt, cam) : nullptr;
+}
 
 struct LpfTab {
     float v[6][64];
@@ -182,29 +193,84 @@ __device__ __forceinline__ CvtMfma<DCT> cvt_mfma(int lane) {
 // selects the 8-point DCT-IV/DST-IV stages: FP32 butterflies (0) or
 // split-precision matrix-core contractions across the wavefront's 8 units
 // (1: FP16 hi + lo, 2: BF16 hi + lo).
+// The loads that start a unit: the tile centre for this camera, then the
+// CltExtra of its kernel cell (a dependent load).  The staged kernel issues
+// them one and two pairs ahead so that their latency overlaps other work.
+struct UnitPre {
+    int task;  // the task word (0: an empty task, nothing stored)
+    int txy;   // tile x | y << 16
+    float cx, cy;
+    float e0, e1, e4, e5, e6, e7;  // CltExtra data_x, data_y, dxc_dx, dxc_dy, dyc_dx, dyc_dy
+};
+
+__device__ __forceinline__ void unit_centre(const ConvertArgs &a, int cam, int task_num, UnitPre &u) {
+    gtask *tk = gtasks(a.tasks) + task_num;
+    u.task = tk->task;
+    u.txy = tk->txy;
+    u.cx = tk->xy[cam][0];
+    u.cy = tk->xy[cam][1];
+}
+
+// kernel cell of a centre, :3129-3134
+__device__ __forceinline__ size_t kernel_cell(const Geo &g, float cx, float cy, int color, float &kdx,
+                                              float &kdy) {
+    int ktx = (int)__builtin_rintf(fmaf(cx, 0.0625f, 1.0f));
+    int kty = (int)__builtin_rintf(fmaf(cy, 0.0625f, 1.0f));
+    ktx = min(g.KH - 1, max(0, ktx));
+    kty = min(g.KV - 1, max(0, kty));
+    kdx = (cx - (float)(ktx << 4)) + 8.0f;
+    kdy = (cy - (float)(kty << 4)) + 8.0f;
+    return (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + color;
+}
+
+template <int COLOR>
+__device__ __forceinline__ void unit_extra(const ConvertArgs &a, int cam, UnitPre &u) {
+    float kdx, kdy;
+    const size_t kfi = kernel_cell(a.g, u.cx, u.cy, COLOR, kdx, kdy);
+    gfloat *ex = gptr(a.l_koffs + kfi * 8);
+    u.e0 = ex[0];
+    u.e1 = ex[1];
+    u.e4 = ex[4];
+    u.e5 = ex[5];
+    u.e6 = ex[6];
+    u.e7 = ex[7];
+}
+
 template <int COLOR, int DCT = 0>
 __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
                                              float *lds_s, int li, int cam, int task_num,
                                              float (&q0)[8], float (&q1)[8], float (&q2)[8],
                                              float (&q3)[8], const CvtMfma<DCT> &mt = {},
-                                             int ustride = 72) {
+                                             int ustride = 72, const UnitPre *pre = nullptr) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
-    gtask *tk = gtasks(a.tasks) + task_num;
-    const float cx = tk->xy[cam][0], cy = tk->xy[cam][1];
+    UnitPre up;
+    if (pre) {
+        up = *pre;
+    } else {
+        unit_centre(a, cam, task_num, up);
+        unit_extra<COLOR>(a, cam, up);
+    }
+    const float cx = up.cx, cy = up.cy;
 
-    // (1) kernel cell, :3129-3134
-    int ktx = (int)__builtin_rintf(fmaf(cx, 0.0625f, 1.0f));
-    int kty = (int)__builtin_rintf(fmaf(cy, 0.0625f, 1.0f));
-    ktx = min(g.KH - 1, max(0, ktx));
-    kty = min(g.KV - 1, max(0, kty));
-    const float kdx = (cx - (float)(ktx << 4)) + 8.0f;
-    const float kdy = (cy - (float)(kty << 4)) + 8.0f;
-    const size_t kfi = (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + COLOR;
-    gfloat *ex = gptr(gelem(a.koffs, cam) + kfi * 8);
-    const float offx = fmaf(ex[5], kdy, fmaf(ex[4], kdx, ex[0]));
-    const float offy = fmaf(ex[7], kdy, fmaf(ex[6], kdx, ex[1]));
-    const float *kern = gelem(a.kernels, cam) + kfi * 256;  // read through gptr4
+    // (1) kernel cell (recomputed from the centre: a few integer operations)
+    float kdx, kdy;
+    const size_t kfi = kernel_cell(g, cx, cy, COLOR, kdx, kdy);
+    const float offx = fmaf(up.e5, kdy, fmaf(up.e4, kdx, up.e0));
+    const float offy = fmaf(up.e7, kdy, fmaf(up.e6, kdx, up.e1));
+    const float *kern = a.l_kern + kfi * 256;  // read through gptr4
+#if TP_KERN_EARLY
+    // this lane's kernel row (4 quadrants x 8 bins), loaded before the
+    // pixels so that its latency overlaps the window and DCT stages
+    f32x4 kpre[8];
+    {
+        gf32x4 *kp = gptr4(kern + li * 8);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
+    }
+#endif
 
     // (2) integer top-left, residual shift, :3182-3247
     const float px = (cx - 8.0f) - offx;
@@ -237,7 +303,7 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // Pixel offsets (32-bit).  Interior windows (the usual case) are linear
     // in the row; windows reaching past the right/bottom edge read the
     // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
-    gfloat *img = gptr(gelem(a.images, cam));
+    gfloat *img = gptr(a.l_img);
     // pixel loads; with TP_IMG_NT they are non-temporal (evict-first in L2),
     // leaving the aberration kernels, which 4 tiles share, resident longer
     auto pix = [&](int o) -> float {
@@ -383,10 +449,16 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827;
     // one half row (4 bins) of the 4 kernel quadrants at a time
     {
+#if !TP_KERN_EARLY
         gf32x4 *kp = gptr4(kern + li * 8);
+#endif
 #pragma unroll
         for (int h = 0; h < 2; h++) {
+#if TP_KERN_EARLY
+            const f32x4 K0 = kpre[h * 4], K1 = kpre[h * 4 + 1], K2 = kpre[h * 4 + 2], K3 = kpre[h * 4 + 3];
+#else
             const f32x4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
+#endif
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
                 const int k = 4 * h + kk;
@@ -472,13 +544,13 @@ __device__ __forceinline__ void store_clt_rows(float *dst, int li, const float (
 template <int COLOR>
 __device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const LaneFold &lf,
                                                    float *slice, int li, int cam, int task_num,
-                                                   bool valid) {
+                                                   bool valid, const UnitPre *pre = nullptr) {
     float q0[8], q1[8], q2[8], q3[8];
-    convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, {}, 256);
+    convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, {}, 256, pre);
     // (12) store [q][l][k], :3600-3630; out-of-grid tiles are skipped
-    const int tile = task_tile(a.g, gtasks(a.tasks)[task_num].txy);
+    const int tile = task_tile(a.g, pre ? pre->txy : gtasks(a.tasks)[task_num].txy);
     const bool st = valid && tile >= 0;
-    float *dst = gelem(a.clt, cam) + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
+    float *dst = a.l_clt + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
 #if TP_CONVERT_LDS_STORE
     // The unit's 1 KB block goes through LDS so that store j writes float4
     // j * 8 + li: every store instruction covers whole 128-byte lines of
@@ -504,6 +576,62 @@ __device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const L
 #endif
 }
 
+// Task of unit slot `unit` in tile pair p of the visiting order (clamped
+// to the last task past the end; `valid` says whether it is a real one).
+__device__ __forceinline__ int pair_task(const ConvertArgs &a, int p, int unit, bool &valid) {
+    const int s = p * 2 + (unit >> 2);
+    valid = s < a.num_tiles;
+    int task_num;
+    if (TP_PANEL_W > 0 && a.num_tiles == a.g.TX * a.g.TY) {
+        // the pair's two tiles from uniform values, then per lane
+        const int sp = __builtin_amdgcn_readfirstlane(p * 2);
+        const int t0 = swizzle_panel(sp, a.g.TX, a.g.TY, TP_PANEL_W);
+        const int t1 = sp + 1 < a.num_tiles ? swizzle_panel(sp + 1, a.g.TX, a.g.TY, TP_PANEL_W) : t0;
+        task_num = (unit >> 2) ? t1 : t0;
+        if (!valid) task_num = a.num_tiles - 1;
+    } else {
+        task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
+    }
+    return task_num;
+}
+
+// The pairs [p0, p1) of one wavefront (one color), software-pipelined: while
+// pair p is converted, the CltExtra of pair p + 1 (whose centre arrived
+// during pair p - 1) and the centre of pair p + 2 are in flight.
+template <int COLOR>
+__device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFold &lf, float *slice,
+                                             int li, int cam, int unit, int p0, int p1) {
+    // The loads are unconditional (past the end they read the last task,
+    // unused) so that the wait counts are the same on every path and a
+    // wait for this pair's data does not also wait for the prefetches.
+    if (p0 >= p1) return;
+    bool v0, v1;
+    int t0 = pair_task(a, p0, unit, v0);
+    int t1 = pair_task(a, p0 + 1, unit, v1);
+    v1 = v1 && p0 + 1 < p1;
+    UnitPre u0, u1;
+    unit_centre(a, cam, t0, u0);
+    unit_centre(a, cam, t1, u1);
+    unit_extra<COLOR>(a, cam, u0);
+    for (int p = p0; p < p1; p++) {
+        bool v2;
+        const int t2 = pair_task(a, p + 2, unit, v2);
+        v2 = v2 && p + 2 < p1;
+        UnitPre u2;
+        unit_centre(a, cam, t2, u2);
+        unit_extra<COLOR>(a, cam, u1);
+        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of this pair's work
+        const bool valid = v0 && u0.task != 0;
+        convert_store_unit<COLOR>(a, lf, slice, li, cam, t0, valid, &u0);
+        u0 = u1;
+        t0 = t1;
+        v0 = v1;
+        u1 = u2;
+        t1 = t2;
+        v1 = v2;
+    }
+}
+
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_correct_tiles(ConvertArgs a) {
     __shared__ __align__(16) float lds[3][8][256];  // [wave][unit]: staging, then output block
@@ -518,28 +646,14 @@ void k_convert_correct_tiles(ConvertArgs a) {
     const int p1 = min(npairs, p0 + a.pairs_per_block);
     float *slice = &lds[wave][unit][0];
     const LaneFold lf = lane_fold(li);
-    const bool panels = TP_PANEL_W > 0 && a.num_tiles == a.g.TX * a.g.TY;
-    for (int p = p0; p < p1; p++) {
-        const int s = p * 2 + (unit >> 2);
-        bool valid = s < a.num_tiles;
-        int task_num;
-        if (panels) {  // the pair's two tiles from uniform values, then per lane
-            const int sp = __builtin_amdgcn_readfirstlane(p * 2);
-            const int t0 = swizzle_panel(sp, a.g.TX, a.g.TY, TP_PANEL_W);
-            const int t1 = sp + 1 < a.num_tiles ? swizzle_panel(sp + 1, a.g.TX, a.g.TY, TP_PANEL_W) : t0;
-            task_num = (unit >> 2) ? t1 : t0;
-            if (!valid) task_num = a.num_tiles - 1;
-        } else {
-            task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
-        }
-        valid = valid && (gtasks(a.tasks)[task_num].task != 0);
-        if (wave == 0)
-            convert_store_unit<0>(a, lf, slice, li, cam, task_num, valid);
-        else if (wave == 1)
-            convert_store_unit<1>(a, lf, slice, li, cam, task_num, valid);
-        else
-            convert_store_unit<2>(a, lf, slice, li, cam, task_num, valid);
-    }
+    ConvertArgs la = a;
+    resolve_cam(la, cam);
+    if (wave == 0)
+        convert_loop<0>(la, lf, slice, li, cam, unit, p0, p1);
+    else if (wave == 1)
+        convert_loop<1>(la, lf, slice, li, cam, unit, p0, p1);
+    else
+        convert_loop<2>(la, lf, slice, li, cam, unit, p0, p1);
 }
 
 }  // namespace tp

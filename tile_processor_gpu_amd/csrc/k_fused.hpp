@@ -162,6 +162,7 @@ void k_fused(FusedArgs a) {
         ca.num_tiles = fr.num_tiles;
         ca.lpf_mask = a.lpf_mask;
         ca.pairs_per_block = 0;
+        resolve_cam(ca, cam);
         const int s0 = (item - (QUEUE ? fr.item_start : 0)) * a.tiles_per_item;
         const int s1 = min(fr.num_tiles, s0 + a.tiles_per_item);
 
@@ -195,7 +196,7 @@ void k_fused(FusedArgs a) {
                 if (fr.clt) {
                     const int tile = task_tile(g, gtasks(fr.tasks)[task_num].txy);
                     if (valid && tile >= 0)
-                        store_clt_rows(gelem(fr.clt, cam) + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
+                        store_clt_rows(ca.l_clt + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
                                        q0, q1, q2, q3);
                 }
             }

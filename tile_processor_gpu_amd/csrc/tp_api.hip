@@ -192,7 +192,7 @@ int tp_convert_correct_tiles(const tp_geometry *geom, float **gpu_kernel_offsets
     if (lpf_mask & ~7) return fail(TP_EINVAL, "tp_convert_correct_tiles: lpf_mask has bits above 2%s", "");
     g_err.clear();
     if (num_tiles == 0) return TP_OK;
-    tp::ConvertArgs a;
+    tp::ConvertArgs a{};
     a.g = tp::make_geo(*geom);
     a.koffs = (const float *const *)gpu_kernel_offsets;
     a.kernels = (const float *const *)gpu_kernels;

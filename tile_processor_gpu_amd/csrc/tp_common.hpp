@@ -31,6 +31,9 @@
 #ifndef TP_FUSED_QWAVES
 #define TP_FUSED_QWAVES 3  // queue form of the FP32 fused kernel (two tiles per group)
 #endif
+#ifndef TP_KERN_EARLY
+#define TP_KERN_EARLY 0    // forward CLT: load the aberration kernel row before the pixels
+#endif
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
