@@ -342,7 +342,7 @@ typedef struct tp_fused_config {
  * images / kernels / kernel_offsets / clt are device arrays of TP_NUM_CAMS
  * device pointers as for tp_convert_correct_tiles; clt may be NULL (CLT not
  * stored), corrs may be NULL (pair tiles not stored), combined may be NULL
- * (no six-pair combination).  item_start is written by the library. */
+ * (no six-pair combination).  num_items is written by the library. */
 typedef struct tp_frame {
     const tp_task *tasks;
     float **images;
@@ -352,7 +352,7 @@ typedef struct tp_frame {
     float *corrs;
     float *combined;
     int32_t num_tiles;
-    int32_t item_start;
+    int32_t num_items;
 } tp_frame;
 
 /* Forward CLT + correlation of one frame.  Arguments as for
@@ -383,15 +383,17 @@ int tp_convert_correlate(const tp_geometry *geom,
                          const tp_fused_config *config,
                          tp_stream_t stream);
 
-#define TP_STREAM_WORK_INTS 288
+#define TP_STREAM_WORK_INTS(nframes) (296 + 8 * (nframes))
 
 /* Forward CLT + correlation of a stream of frames (each with its own task
  * list, images, kernels and outputs) in one persistent launch: a fixed grid
  * of blocks_per_cu workgroups per CU takes work items from 8 per-XCD queues
- * (contiguous eighths of the items, with stealing).
- * gpu_work: device int32[TP_STREAM_WORK_INTS], reset by the call: the 8 queue
- * counters at [32 x] (one 128-byte line each, so that the claims of
- * different XCDs do not contend for one line) and the item count at [256].
+ * (queue x: the x-th eighth of every frame's items, frame after frame, so
+ * that all XCDs work on the same frames at a time; idle workgroups steal).
+ * gpu_work: device int32[TP_STREAM_WORK_INTS(nframes)], written by the call:
+ * the 8 queue counters at [32 x] (one 128-byte line each, so that the claims
+ * of different XCDs do not contend for one line), the item count at [256],
+ * and from [288] a [nframes + 1][8] table of per-queue item prefixes.
  * gpu_stamps: NULL or device uint64[nframes][2], set to each frame's first
  * item start and last item end (GPU wall clock, wall_clock_hz ticks/s). */
 int tp_stream_convert_correlate(const tp_geometry *geom,

@@ -20,9 +20,10 @@
 // Work items are runs of tiles_per_item consecutive tiles (in the visiting
 // order of swizzle_strip) of one frame.  Single-frame launches map one
 // workgroup to one item.  Stream launches (QUEUE) are persistent: a fixed
-// grid of workgroups takes items from 8 per-XCD queues (contiguous eighths
-// of all items, with stealing) until every frame's items are done; frames
-// are located by binary search on item_start.
+// grid of workgroups takes items from 8 per-XCD queues (queue x holds the
+// x-th eighth of each frame's items, frame after frame; idle workgroups
+// steal from the other queues) until every frame's items are done; a claim
+// is mapped to its frame by binary search in a per-queue prefix table.
 #pragma once
 
 #include "k_convert.hpp"
@@ -33,7 +34,12 @@ namespace tp {
 
 constexpr int TP_WORK_LINE = 32;    // ints per 128-byte line: one claim counter per line
 constexpr int TP_WORK_ITEMS = 256;  // item count, after the 8 counter lines
-static_assert(TP_WORK_ITEMS < TP_STREAM_WORK_INTS, "stream work buffer");
+constexpr int TP_WORK_TABLE = 288;  // then [nframes + 1][8]: per-XCD item prefix over frames
+static_assert(TP_WORK_TABLE + 8 == TP_STREAM_WORK_INTS(0), "stream work buffer");
+
+// Items of frame f (n items) in XCD x's queue: the x-th eighth of the frame,
+// [n x / 8, n (x + 1) / 8).
+__device__ __forceinline__ int eighth_lo(int n, int x) { return (int)((long long)n * x / 8); }
 
 // Binary-identical to tp_frame (include/tp.h).
 struct FrameDesc {
@@ -45,7 +51,7 @@ struct FrameDesc {
     float *corrs;       // row 6 * task + pair, may be null
     float *combined;    // row task: the six-pair combination, may be null
     int num_tiles;
-    int item_start;
+    int num_items;
 };
 
 struct FusedArgs {
@@ -84,7 +90,7 @@ void k_fused(FusedArgs a) {
     // [tile][camera][color][4 x 8 x 8]; a unit's slice also serves as its
     // transposition staging (2 x 72 floats) before its CLT is written there
     __shared__ __align__(16) float clt_lds[GT * 4 * 3 * 256];
-    __shared__ int s_item;
+    __shared__ int s_item, s_frame;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -96,45 +102,50 @@ void k_fused(FusedArgs a) {
     corr_out_offsets(lane, a.radius, src_off);
     const float fz2 = a.fat_zero * a.fat_zero;
     const Geo &g = a.g;
-    const int total = QUEUE ? a.work[TP_WORK_ITEMS] : (int)gridDim.x;
     int item = QUEUE ? 0 : xcd_remap(blockIdx.x, gridDim.x);
+    // frame descriptors live in global memory (scalar loads, not flat ones)
+    typedef __attribute__((address_space(1))) const FrameDesc gframe;
+    gframe *frames = (gframe *)a.frames;
+    typedef __attribute__((address_space(1))) const int gint;
+    gint *table = (gint *)a.work + TP_WORK_TABLE;
 
     while (true) {
+        int f = 0;
         if (QUEUE) {
-            // One queue per XCD over a contiguous eighth of the items, so
-            // that neighbouring tiles (which share kernel cells and image
-            // rows) are converted under one L2; a workgroup (on XCD
-            // blockIdx.x % 8) drains its own queue first, then the others.
+            // One queue per XCD: the x-th eighth of every frame, frame after
+            // frame.  Neighbouring tiles (which share kernel cells and image
+            // rows) are converted under one L2, and all XCDs work on the
+            // same frames at a time, so a frame finishes soon after it
+            // starts.  A workgroup (on XCD blockIdx.x % 8) drains its own
+            // queue first, then the others.
             if (threadIdx.x == 0) {
-                int it = -1;
+                int it = -1, fi = 0;
                 const int x0 = blockIdx.x & 7;
                 for (int k = 0; k < 8 && it < 0; k++) {
                     const int x = (x0 + k) & 7;
-                    const int lo = (int)((long long)total * x / 8);
-                    const int hi = (int)((long long)total * (x + 1) / 8);
                     const int c = atomicAdd(&a.work[x * TP_WORK_LINE], 1);
-                    if (c < hi - lo) it = lo + c;
+                    if (c < table[a.nframes * 8 + x]) {
+                        // last frame whose prefix for queue x is <= c
+                        int lo = 0, hi = a.nframes - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (table[mid * 8 + x] <= c) lo = mid; else hi = mid - 1;
+                        }
+                        const int n = frames[lo].num_items;
+                        it = eighth_lo(n, x) + (c - table[lo * 8 + x]);
+                        fi = lo;
+                    }
                 }
                 s_item = it;
+                s_frame = fi;
             }
             __syncthreads();
             // the same value in every lane: keep it (and the frame descriptor
             // and pointers derived from it) in scalar registers
             item = __builtin_amdgcn_readfirstlane(s_item);
+            f = __builtin_amdgcn_readfirstlane(s_frame);
             __syncthreads();  // s_item is rewritten by the next iteration
             if (item < 0) break;  // every queue is drained
-        }
-        int f = 0;
-        // frame descriptors live in global memory (scalar loads, not flat ones)
-        typedef __attribute__((address_space(1))) const FrameDesc gframe;
-        gframe *frames = (gframe *)a.frames;
-        if (QUEUE) {  // last frame with item_start <= item
-            int lo = 0, hi = a.nframes - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (frames[mid].item_start <= item) lo = mid; else hi = mid - 1;
-            }
-            f = lo;
         }
         FrameDesc fr = a.f0;
         if (QUEUE) {
@@ -147,7 +158,7 @@ void k_fused(FusedArgs a) {
             fr.corrs = d.corrs;
             fr.combined = d.combined;
             fr.num_tiles = d.num_tiles;
-            fr.item_start = d.item_start;
+            fr.num_items = d.num_items;
         }
         if (QUEUE && a.stamps && threadIdx.x == 0)
             atomicMin(&a.stamps[2 * f], (unsigned long long)wall_clock64());
@@ -163,7 +174,7 @@ void k_fused(FusedArgs a) {
         ca.lpf_mask = a.lpf_mask;
         ca.pairs_per_block = 0;
         resolve_cam(ca, cam);
-        const int s0 = (item - (QUEUE ? fr.item_start : 0)) * a.tiles_per_item;
+        const int s0 = item * a.tiles_per_item;  // item: index within the frame
         const int s1 = min(fr.num_tiles, s0 + a.tiles_per_item);
 
         for (int gs = s0; gs < s1; gs += GT) {
@@ -275,18 +286,27 @@ void k_fused(FusedArgs a) {
     }
 }
 
-// Stream set-up: item_start of every frame, the queue counter and the
-// per-frame time stamps.  One wavefront; frames are few.
+// Stream set-up: each frame's item count, the per-XCD item
+// prefix table over frames, the queue counters and the per-frame time
+// stamps.  One wavefront; lane x < 8 fills column x of the table.
 __global__ void k_stream_setup(FrameDesc *frames, int nframes, int tiles_per_item, int *work,
                                unsigned long long *stamps) {
-    if (threadIdx.x == 0) {
+    const int x = threadIdx.x;
+    if (x < 8) {
         int acc = 0;
         for (int f = 0; f < nframes; f++) {
-            frames[f].item_start = acc;
-            acc += (max(frames[f].num_tiles, 0) + tiles_per_item - 1) / tiles_per_item;
+            const int n = (max(frames[f].num_tiles, 0) + tiles_per_item - 1) / tiles_per_item;
+            if (x == 0) frames[f].num_items = n;
+            work[TP_WORK_TABLE + f * 8 + x] = acc;
+            acc += eighth_lo(n, x + 1) - eighth_lo(n, x);
         }
-        for (int x = 0; x < 8; x++) work[x * TP_WORK_LINE] = 0;  // per-XCD claim counters
-        work[TP_WORK_ITEMS] = acc;                              // number of items
+        work[TP_WORK_TABLE + nframes * 8 + x] = acc;  // queue x's length
+        work[x * TP_WORK_LINE] = 0;                    // claim counter
+    }
+    if (x == 0) {
+        int total = 0;
+        for (int f = 0; f < nframes; f++) total += frames[f].num_items;
+        work[TP_WORK_ITEMS] = total;  // number of items
     }
     if (stamps)
         for (int i = threadIdx.x; i < nframes; i += blockDim.x) {

@@ -55,7 +55,7 @@ inline hipStream_t as_stream(tp_stream_t s) { return (hipStream_t)s; }
 static_assert(sizeof(tp_frame) == sizeof(tp::FrameDesc), "tp_frame layout");
 static_assert(TP_NN_PARAMS == tp::NN_NPARAMS && TP_NN_ACTS == tp::NN_ACTS &&
               TP_NN_INPUT == tp::NN_IN * tp::NN_P, "CNN sizes");
-static_assert(offsetof(tp_frame, item_start) == offsetof(tp::FrameDesc, item_start), "tp_frame layout");
+static_assert(offsetof(tp_frame, num_items) == offsetof(tp::FrameDesc, num_items), "tp_frame layout");
 
 struct FusedCfg {
     int tpi, gt, bpc, dct;
@@ -562,7 +562,7 @@ int tp_convert_correlate(const tp_geometry *geom, float **gpu_kernel_offsets, fl
     a.f0.corrs = gpu_corrs;
     a.f0.combined = gpu_combined;
     a.f0.num_tiles = num_tiles;
-    a.f0.item_start = 0;
+    a.f0.num_items = 0;
     launch_fused<false>(a, c, colors, (num_tiles + c.tpi - 1) / c.tpi, as_stream(stream));
     return check_launch("fused");
 }

@@ -139,10 +139,14 @@ def convert_correlate(geom, gpu_kernel_offsets, gpu_kernels, gpu_images, gpu_tas
 
 
 # tp_frame (include/tp.h): one frame of the stream form
-STREAM_WORK_INTS = 288  # TP_STREAM_WORK_INTS
+def stream_work_ints(nframes):
+    """TP_STREAM_WORK_INTS(nframes): int32 words of the stream work buffer."""
+    return 296 + 8 * nframes
+
+
 FRAME_DTYPE = np.dtype([("tasks", "<u8"), ("images", "<u8"), ("kernels", "<u8"),
                         ("kernel_offsets", "<u8"), ("clt", "<u8"), ("corrs", "<u8"),
-                        ("combined", "<u8"), ("num_tiles", "<i4"), ("item_start", "<i4")])
+                        ("combined", "<u8"), ("num_tiles", "<i4"), ("num_items", "<i4")])
 
 
 def wall_clock_hz():
@@ -157,8 +161,9 @@ class StreamRun:
         """True when every per-XCD queue has handed out all its items (call
         after synchronising)."""
         w = self.work.cpu().numpy().astype(np.int64)
-        total = int(w[256])
-        return all(w[32 * x] >= total * (x + 1) // 8 - total * x // 8 for x in range(8))
+        nf = len(self.frames)
+        lengths = w[288 + 8 * nf:288 + 8 * nf + 8]  # per-queue item counts
+        return int(lengths.sum()) == int(w[256]) and all(w[32 * x] >= lengths[x] for x in range(8))
 
     def __init__(self, frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, lpf_mask=0,
                  store_clt=False, config=None, store_pairs=True, combined=False, pair_mask=0x3F,
@@ -186,7 +191,7 @@ class StreamRun:
         self.pair_mask = pair_mask
         self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
         # per-XCD queue counters (one 128-byte line each) and the item count
-        self.work = torch.zeros(STREAM_WORK_INTS, dtype=torch.int32, device=dev)
+        self.work = torch.zeros(stream_work_ints(len(frames)), dtype=torch.int32, device=dev)
         # per-frame GPU time stamps (two atomics per work item on the frame's words)
         self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev) if stamps else None
         self.graph = None
