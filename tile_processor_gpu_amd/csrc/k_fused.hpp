@@ -73,13 +73,13 @@ struct FusedArgs {
 };
 
 // Occupancy: the FP32-DCT kernel fits 168 VGPRs, i.e. 3 waves per SIMD;
-// left to itself the compiler takes a few more registers in the queue form
-// and drops to 2 waves per SIMD, 1.2-1.6x slower, so the queue form asks
-// for 3.  The matrix-core forms need more than 168 registers and keep the
+// left to itself the compiler takes a few more registers in some forms and
+// drops to 2 waves per SIMD, 1.2-1.6x slower, so the FP32 forms ask for 3.
+// The matrix-core forms need more than 168 registers and keep the
 // compiler's choice.
 template <int GT, bool QUEUE, int COLORS, int DCT>
 constexpr int fused_min_waves() {
-    return (DCT == 0 && QUEUE) ? TP_FUSED_QWAVES : TP_FUSED_WAVES;
+    return DCT == 0 ? TP_FUSED_QWAVES : TP_FUSED_WAVES;
 }
 
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>

@@ -69,22 +69,52 @@ __device__ __forceinline__ float alpha_fade(int m, int i, int j) {
     }
 }
 
-// LDS of one texture tile
+// LDS of one texture tile.  M planes are TEX_MS floats apart: the imclt
+// unfold writes columns {0-3, 8-11} (or {4-7, 12-15}) of one row in each of
+// 8 planes at once, and the 4-float skew spreads those over all 32 banks.
+constexpr int TEX_MS = 260;
 struct TexShared {
-    float M[12 * 256];  // [cam * colors + color][256]
+    float M[12 * TEX_MS];  // [cam * colors + color][256]
     union {
-        float scratch[4][8][4][72];  // phase 1: imclt transposes
-        struct {
-            float tmp[12 * 256];
-            float D[12 * 256];
-        } p2;
+        float scratch[2][8][4][72];  // phase 1: imclt transposes (wavefronts 0, 1)
+        float D[12 * 256];           // phase 2: debayered planes
     } u;
 };
+
+// Per-pixel constants of thread t (pixel r = t / 16, c = t % 16), the same
+// for every tile: the window weights and the 3 x 3 debayer neighbourhood
+// (offsets, and the un-window factors of each neighbour; edge pixels use
+// their own value only).
+struct TexPix {
+    int r, c;
+    bool edge;
+    float wsq, wnd2_inv;
+    int nb[9];    // plane offsets of the neighbours (dr, dc) = (k / 3 - 1, k % 3 - 1)
+    float fw[9];  // 1 / window^2 of each neighbour
+};
+
+__device__ __forceinline__ TexPix tex_pix(int t) {
+    TexPix q;
+    q.r = t >> 4;
+    q.c = t & 15;
+    auto sym = [](int i) { return i < 8 ? i : 15 - i; };
+    q.wsq = hw_sq(sym(q.r)) * hw_sq(sym(q.c));
+    q.wnd2_inv = 1.0f / q.wsq;
+    q.edge = q.r == 0 || q.r == 15 || q.c == 0 || q.c == 15;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int dr = q.edge ? 0 : k / 3 - 1, dc = q.edge ? 0 : k % 3 - 1;
+        q.nb[k] = t + 16 * dr + dc;
+        q.fw[k] = (1.0f / hw_sq(sym(q.c + dc))) * (1.0f / hw_sq(sym(q.r + dr)));
+    }
+    return q;
+}
 
 // One texture list entry e (of n) by the whole workgroup; returns early
 // (uniformly) for entries without a texture bit or off the grid.
 template <int COLORS>
-__device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, TexShared &sh) {
+__device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, TexShared &sh,
+                                             const TexPix &px) {
     float *M = sh.M;
     auto &u = sh.u;
     const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
@@ -108,40 +138,38 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
             q0[k] *= l; q1[k] *= l; q2[k] *= l; q3[k] *= l;
         }
         imclt_transform(q0, q1, q2, q3, &u.scratch[wave][lane >> 3][0][0], li);
-        float *m = M + slot * 256;
+        float *m = M + slot * TEX_MS;
         imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
     }
     __syncthreads();
 
-    const int t = threadIdx.x, r = t >> 4, c = t & 15;
-    const int rs = r < 8 ? r : 15 - r, cs = c < 8 ? c : 15 - c;
-    const float wsq = hw_sq(rs) * hw_sq(cs);
-    // phase 2: debayer_shot (colors > 1), else D = M
+    const int t = threadIdx.x, r = px.r, c = px.c;
+    const float wsq = px.wsq;
+    // phase 2: debayer_shot (colors > 1): un-window, 3 x 3 debayer (edge
+    // pixels unchanged), shot-noise companding, re-window; else D = M
     if (colors > 1) {
-        const float wsqi_r = 1.0f / hw_sq(rs), wsqi_c = 1.0f / hw_sq(cs);
-        for (int s = 0; s < 4 * colors; s++) u.p2.tmp[s * 256 + t] = M[s * 256 + t] * wsqi_c * wsqi_r;
-        __syncthreads();
         const float k = 1.0f / sqrtf(a.min_shot);
+#pragma unroll
         for (int s = 0; s < 4 * colors; s++) {
-            const float *x = u.p2.tmp + s * 256;
+            const float *x = M + s * TEX_MS;
             const int col = s % colors;
+            float v[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) v[j] = x[px.nb[j]] * px.fw[j];
+            const float cross = (v[1] + v[7]) + (v[3] + v[5]);
             float d;
-            if (r == 0 || r == 15 || c == 0 || c == 15) {
-                d = x[t];
+            if (col < 2) {
+                const float diag = (v[0] + v[2]) + (v[6] + v[8]);
+                d = fmaf(0.0625f, diag, fmaf(0.125f, cross, 0.25f * v[4]));
             } else {
-                const float cross = (x[t - 16] + x[t + 16]) + (x[t - 1] + x[t + 1]);
-                if (col < 2) {
-                    const float diag = (x[t - 17] + x[t - 15]) + (x[t + 15] + x[t + 17]);
-                    d = fmaf(0.0625f, diag, fmaf(0.125f, cross, 0.25f * x[t]));
-                } else {
-                    d = fmaf(0.125f, cross, 0.5f * x[t]);
-                }
+                d = fmaf(0.125f, cross, 0.5f * v[4]);
             }
+            d = px.edge ? v[4] : d;
             if (a.scale_shot > 0.0f) d = a.scale_shot * (d > a.min_shot ? sqrtf(d) : k * d);
-            u.p2.D[s * 256 + t] = d * wsq;
+            u.D[s * 256 + t] = d * wsq;
         }
     } else {
-        for (int s = 0; s < 4; s++) u.p2.D[s * 256 + t] = M[s * 256 + t];
+        for (int s = 0; s < 4; s++) u.D[s * 256 + t] = M[s * TEX_MS + t];
     }
     __syncthreads();
 
@@ -151,11 +179,11 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
     for (int cam = 0; cam < 4; cam++)
 #pragma unroll
         for (int col = 0; col < 3; col++) {
-            D[cam][col] = col < colors ? u.p2.D[(cam * colors + col) * 256 + t] : 0.f;
-            Mv[cam][col] = col < colors ? M[(cam * colors + col) * 256 + t] : 0.f;
+            D[cam][col] = col < colors ? u.D[(cam * colors + col) * 256 + t] : 0.f;
+            Mv[cam][col] = col < colors ? M[(cam * colors + col) * TEX_MS + t] : 0.f;
         }
     const float ksigma = 1.0f / (2.0f * a.diff_sigma * a.diff_sigma);
-    const float wnd2_inv = 1.0f / wsq;
+    const float wnd2_inv = px.wnd2_inv;
     float out[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) out[i] = 0.f;
@@ -191,7 +219,7 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
                 const float dc = (D[i][col] - D[j][col]) * wnd2_inv;
                 d = fmaf(a.w[col] * dc, dc, d);
             }
-            d = expf(-dist2r * d) + (float)TP_FAT_ZERO_WEIGHT;
+            d = __expf(-dist2r * d) + (float)TP_FAT_ZERO_WEIGHT;
             p[i] += d;
             p[j] += d;
         }
@@ -226,7 +254,7 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
             const float dc = (D[cam][col] - blend[col]) * wnd2_inv;
             d2 = fmaf(a.w[col] * dc, dc, d2);
         }
-        p[cam] = expf(-ksigma * d2) + (float)TP_FAT_ZERO_WEIGHT;
+        p[cam] = __expf(-ksigma * d2) + (float)TP_FAT_ZERO_WEIGHT;
     }
     if (a.dust_remove) {
         int worst = 0;
@@ -272,8 +300,9 @@ template <int COLORS>
 __global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
     __shared__ __align__(16) TexShared sh;
     const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
+    const TexPix px = tex_pix(threadIdx.x);
     for (int e = blockIdx.x; e < n; e += gridDim.x) {
-        texture_tile<COLORS>(a, e, n, sh);
+        texture_tile<COLORS>(a, e, n, sh, px);
         __syncthreads();  // the LDS of this entry is rewritten by the next one
     }
 }

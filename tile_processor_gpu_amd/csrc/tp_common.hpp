@@ -29,7 +29,7 @@
 #define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
 #endif
 #ifndef TP_FUSED_QWAVES
-#define TP_FUSED_QWAVES 3  // queue form of the FP32 fused kernel (two tiles per group)
+#define TP_FUSED_QWAVES 3  // FP32 fused kernel: minimum waves per SIMD
 #endif
 #ifndef TP_KERN_EARLY
 #define TP_KERN_EARLY 0    // forward CLT: load the aberration kernel row before the pixels
