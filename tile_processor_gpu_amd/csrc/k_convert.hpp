@@ -90,6 +90,23 @@ __device__ __forceinline__ int swizzle_strip(int s, int TX, int num_tiles) {
     return st * strip + r * TX + c;
 }
 
+// Group order (staged forward CLT): strips of SH tile rows, each cut into
+// groups of CW columns, a group visited row by row.  With CW = 4 and
+// 8 tiles per workgroup, a workgroup converts 2 rows x 4 columns: whole
+// 48 KB runs of the CLT rows per camera and color plane, as in the
+// column-pair order, while the SH / 2 workgroups of a group (which run at
+// the same time on one XCD) share the aberration kernel rows between them.
+// Identity past the last complete strip.  A bijection on [0, num_tiles).
+__device__ __forceinline__ int swizzle_group(int s, int TX, int num_tiles, int SH, int CW) {
+    const int strip = SH * TX;
+    if (s >= (num_tiles / strip) * strip) return s;
+    const int st = s / strip, q = s - st * strip;
+    const int g = q / (SH * CW);
+    const int c0 = g * CW, w = min(CW, TX - c0);
+    const int r = q - g * SH * CW;
+    return st * strip + (r / w) * TX + c0 + r % w;
+}
+
 // Panel order for a full raster task list (num_tiles == TX * TY): panels of
 // PW tile columns, each visited as swizzle_strip visits the whole grid
 // (strips of TP_STRIP_ROWS rows, column pairs top to bottom), so that the
@@ -186,29 +203,40 @@ __device__ __forceinline__ CvtMfma<DCT> cvt_mfma(int lane) {
     return t;
 }
 
-// One unit (tile task_num, camera cam, color COLOR) up to the optional LPF:
-// on return lane li holds row l = li of the 4 quadrants in q0..q3.  lds_c,
-// lds_s: this unit's two 8 x 8 LDS staging blocks (the 8 units of a
-// wavefront are ustride floats apart).  DCT
-// selects the 8-point DCT-IV/DST-IV stages: FP32 butterflies (0) or
-// split-precision matrix-core contractions across the wavefront's 8 units
-// (1: FP16 hi + lo, 2: BF16 hi + lo).
 // The loads that start a unit: the tile centre for this camera, then the
 // CltExtra of its kernel cell (a dependent load).  The staged kernel issues
 // them one and two pairs ahead so that their latency overlaps other work.
+// Each of the unit's 8 lanes loads one word of a record (the vector memory
+// path is the forward CLT's busiest unit, and it is paid per lane and
+// byte), and the words are broadcast across the unit's lanes with
+// ds_swizzle where they are used.
 struct UnitPre {
     int task;  // the task word (0: an empty task, nothing stored)
     int txy;   // tile x | y << 16
     float cx, cy;
-    float e0, e1, e4, e5, e6, e7;  // CltExtra data_x, data_y, dxc_dx, dxc_dy, dyc_dx, dyc_dy
+    int wc;    // this lane's word of the task record (lane li & 3: task, txy, x, y)
+    int we;    // this lane's word of the CltExtra record (word li)
 };
 
+// Word K of the lane's 8-lane unit (lane (lane & ~7) | K): swizzle in
+// bit-mask mode, and_mask 0x18, or_mask K.
+template <int K>
+__device__ __forceinline__ int unit_word(int v) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (K << 5));
+}
+
 __device__ __forceinline__ void unit_centre(const ConvertArgs &a, int cam, int task_num, UnitPre &u) {
-    gtask *tk = gtasks(a.tasks) + task_num;
-    u.task = tk->task;
-    u.txy = tk->txy;
-    u.cx = tk->xy[cam][0];
-    u.cy = tk->xy[cam][1];
+    const int li = __lane_id() & 3;
+    const int w = li < 2 ? li : 2 * cam + li;  // task, txy, xy[cam][0], xy[cam][1]
+    u.wc = ((__attribute__((address_space(1))) const int *)(gtasks(a.tasks) + task_num))[w];
+}
+
+// task, txy and centre from the loaded task words
+__device__ __forceinline__ void unit_centre_bcast(UnitPre &u) {
+    u.task = unit_word<0>(u.wc);
+    u.txy = unit_word<1>(u.wc);
+    u.cx = __int_as_float(unit_word<2>(u.wc));
+    u.cy = __int_as_float(unit_word<3>(u.wc));
 }
 
 // kernel cell of a centre, :3129-3134
@@ -223,54 +251,45 @@ __device__ __forceinline__ size_t kernel_cell(const Geo &g, float cx, float cy, 
     return (size_t)(ktx + kty * g.KH) * TP_NUM_COLORS + color;
 }
 
+// Broadcasts the task words, then loads this lane's word of the CltExtra of
+// the unit's kernel cell.
 template <int COLOR>
 __device__ __forceinline__ void unit_extra(const ConvertArgs &a, int cam, UnitPre &u) {
+    unit_centre_bcast(u);
     float kdx, kdy;
     const size_t kfi = kernel_cell(a.g, u.cx, u.cy, COLOR, kdx, kdy);
-    gfloat *ex = gptr(a.l_koffs + kfi * 8);
-    u.e0 = ex[0];
-    u.e1 = ex[1];
-    u.e4 = ex[4];
-    u.e5 = ex[5];
-    u.e6 = ex[6];
-    u.e7 = ex[7];
+    u.we = __float_as_int(gptr(a.l_koffs + kfi * 8)[__lane_id() & 7]);
 }
 
-template <int COLOR, int DCT = 0>
-__device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
-                                             float *lds_s, int li, int cam, int task_num,
-                                             float (&q0)[8], float (&q1)[8], float (&q2)[8],
-                                             float (&q3)[8], const CvtMfma<DCT> &mt = {},
-                                             int ustride = 72, const UnitPre *pre = nullptr) {
+// Phase 1 of a unit: kernel cell, integer window position and residual
+// shift, and the unit's Bayer samples (R, B: 8 per lane; G: 16).  The sample
+// loads are issued here and first used by unit_compute, so a caller can issue
+// them early (the staged kernel issues the next unit's ahead of this unit's
+// stores: loads and stores share one completion counter, so a wait for loads
+// issued after stores is also a wait for the stores).
+struct UnitFetch {
+    float p[16];     // R, B: p[l] = sample of folded row l; G: p[l] from row ra, p[8 + l] from row rb
+    float sh, sv;    // residual shifts (px)
+    int itlx, itly;  // window top-left
+    int kfi;         // kernel cell * 3 + color
+};
+
+template <int COLOR>
+__device__ __forceinline__ void unit_fetch(const ConvertArgs &a, const LaneFold &lf, const UnitPre &up,
+                                           UnitFetch &F) {
     constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
     const Geo &g = a.g;
-    UnitPre up;
-    if (pre) {
-        up = *pre;
-    } else {
-        unit_centre(a, cam, task_num, up);
-        unit_extra<COLOR>(a, cam, up);
-    }
     const float cx = up.cx, cy = up.cy;
 
     // (1) kernel cell (recomputed from the centre: a few integer operations)
     float kdx, kdy;
-    const size_t kfi = kernel_cell(g, cx, cy, COLOR, kdx, kdy);
-    const float offx = fmaf(up.e5, kdy, fmaf(up.e4, kdx, up.e0));
-    const float offy = fmaf(up.e7, kdy, fmaf(up.e6, kdx, up.e1));
-    const float *kern = a.l_kern + kfi * 256;  // read through gptr4
-#if TP_KERN_EARLY
-    // this lane's kernel row (4 quadrants x 8 bins), loaded before the
-    // pixels so that its latency overlaps the window and DCT stages
-    f32x4 kpre[8];
-    {
-        gf32x4 *kp = gptr4(kern + li * 8);
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
-    }
-#endif
+    F.kfi = (int)kernel_cell(g, cx, cy, COLOR, kdx, kdy);
+    // CltExtra data_x, data_y, dxc_dx, dxc_dy, dyc_dx, dyc_dy
+    const float e0 = __int_as_float(unit_word<0>(up.we)), e1 = __int_as_float(unit_word<1>(up.we));
+    const float e4 = __int_as_float(unit_word<4>(up.we)), e5 = __int_as_float(unit_word<5>(up.we));
+    const float e6 = __int_as_float(unit_word<6>(up.we)), e7 = __int_as_float(unit_word<7>(up.we));
+    const float offx = fmaf(e5, kdy, fmaf(e4, kdx, e0));
+    const float offy = fmaf(e7, kdy, fmaf(e6, kdx, e1));
 
     // (2) integer top-left, residual shift, :3182-3247
     const float px = (cx - 8.0f) - offx;
@@ -281,24 +300,10 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     if (itlx >= g.W - 8) itlx = (itlx & 1) + (g.W - 10);
     if (itly < 0) itly &= 1;
     if (itly >= g.H - 8) itly = (itly & 1) + (g.H - 10);
-    const float sh = ((float)(itlx + 8) - cx) + offx;
-    const float sv = ((float)(itly + 8) - cy) + offy;
-    float ahs, ahc, avs, avc;
-    sincospif(sh * 0.0625f, &ahs, &ahc);
-    sincospif(sv * 0.0625f, &avs, &avc);
-
-    // (3) windows.  Vertical: all 16 source rows (compile-time n); the fold
-    // sign (cosine branch) is folded in.
-    float wv[16];
-#pragma unroll
-    for (int n = 0; n < 16; n++) {
-        const float P = n < 8 ? kTab.hw[n] : kTab.hw[15 - n];
-        const float Q = n < 8 ? kTab.hw[7 - n] : -kTab.hw[n - 8];
-        wv[n] = (float)fold_rcs(n) * fmaf(P, avc, Q * avs);
-    }
-    // Horizontal: this lane's two source columns, by local parity.
-    const float whc0 = fmaf(lf.Pc0, ahc, lf.Qc0 * ahs), whc1 = fmaf(lf.Pc1, ahc, lf.Qc1 * ahs);
-    const float whs0 = fmaf(lf.Ps0, ahc, lf.Qs0 * ahs), whs1 = fmaf(lf.Ps1, ahc, lf.Qs1 * ahs);
+    F.sh = ((float)(itlx + 8) - cx) + offx;
+    F.sv = ((float)(itly + 8) - cy) + offy;
+    F.itlx = itlx;
+    F.itly = itly;
 
     // Pixel offsets (32-bit).  Interior windows (the usual case) are linear
     // in the row; windows reaching past the right/bottom edge read the
@@ -324,48 +329,94 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
         return inside ? x : min(x, g.W - 2 + (x & 1));
     };
     const int x0 = coloff(lf.col0), x1 = coloff(lf.col1);
+    if (!color1) {
+        const int pc = (itlx & 1) ^ (1 ^ color0);
+        const int pr = (itly & 1) ^ color0;
+        const int xo = pc ? x1 : x0;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            F.p[l] = pix(rowoff((ra & 1) == pr ? ra : rb) + xo);
+        }
+    } else {
+        // G sites: (row + col + itlx + itly) even; both source rows used
+        const int S = (itlx + itly) & 1;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
+            const int pca = (ra + S) & 1;  // column parity for row ra; row rb takes the other
+            F.p[l] = pix(rowoff(ra) + (pca ? x1 : x0));
+            F.p[8 + l] = pix(rowoff(rb) + (pca ? x0 : x1));
+        }
+    }
+}
+
+// Phase 2 of a unit: windows, fold, DCTs, replication, kernel, phase shift.
+// On return lane li holds row l = li of the 4 quadrants in q0..q3.  lds_c,
+// lds_s: this unit's two 8 x 8 LDS staging blocks (the 8 units of a
+// wavefront are ustride floats apart).  DCT selects the 8-point
+// DCT-IV/DST-IV stages: FP32 butterflies (0) or split-precision matrix-core
+// contractions across the wavefront's 8 units (1: FP16 hi + lo, 2: BF16
+// hi + lo).
+template <int COLOR, int DCT = 0>
+__device__ __forceinline__ void unit_compute(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
+                                             float *lds_s, int li, const UnitFetch &F,
+                                             float (&q0)[8], float (&q1)[8], float (&q2)[8],
+                                             float (&q3)[8], const CvtMfma<DCT> &mt = {},
+                                             int ustride = 72) {
+    constexpr int color0 = COLOR & 1, color1 = (COLOR >> 1) & 1;
+    const int itlx = F.itlx, itly = F.itly;
+    const float sh = F.sh, sv = F.sv;
+    const float *kern = a.l_kern + (size_t)F.kfi * 256;  // read through gptr4
+#if TP_KERN_EARLY
+    // this lane's kernel row (4 quadrants x 8 bins), loaded before the
+    // window and DCT stages so that its latency overlaps them
+    f32x4 kpre[8];
+    {
+        gf32x4 *kp = gptr4(kern + li * 8);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
+    }
+#endif
+    float ahs, ahc, avs, avc;
+    sincospif(sh * 0.0625f, &ahs, &ahc);
+    sincospif(sv * 0.0625f, &avs, &avc);
+
+    // (3) windows.  Vertical: all 16 source rows (compile-time n); the fold
+    // sign (cosine branch) is folded in.
+    float wv[16];
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+        const float P = n < 8 ? kTab.hw[n] : kTab.hw[15 - n];
+        const float Q = n < 8 ? kTab.hw[7 - n] : -kTab.hw[n - 8];
+        wv[n] = (float)fold_rcs(n) * fmaf(P, avc, Q * avs);
+    }
+    // Horizontal: this lane's two source columns, by local parity.
+    const float whc0 = fmaf(lf.Pc0, ahc, lf.Qc0 * ahs), whc1 = fmaf(lf.Pc1, ahc, lf.Qc1 * ahs);
+    const float whs0 = fmaf(lf.Ps0, ahc, lf.Qs0 * ahs), whs1 = fmaf(lf.Ps1, ahc, lf.Qs1 * ahs);
 
     // (4) fold: f[l] = folded (l, li)
     float f[8], fs[8];
     if (!color1) {
         const int pc = (itlx & 1) ^ (1 ^ color0);
         const int pr = (itly & 1) ^ color0;
-        const int xo = pc ? x1 : x0;
         const float wc = pc ? whc1 : whc0;
-        // all loads are issued before the first use (the scheduling barrier
-        // keeps the compiler from pairing each load with its use, which
-        // serialises the HBM latencies when it is short of registers)
-        float px[8];
-#pragma unroll
-        for (int l = 0; l < 8; l++) {
-            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
-            px[l] = pix(rowoff((ra & 1) == pr ? ra : rb) + xo);
-        }
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
             const float w = (ra & 1) == pr ? wv[ra] : wv[rb];
-            f[l] = px[l] * w * wc;
+            f[l] = F.p[l] * w * wc;
         }
     } else {
-        // G sites: (row + col + itlx + itly) even; both source rows used
         const int S = (itlx + itly) & 1;
-        float pa[8], pb[8];
-#pragma unroll
-        for (int l = 0; l < 8; l++) {
-            const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
-            const int pca = (ra + S) & 1;  // column parity for row ra; row rb takes the other
-            pa[l] = pix(rowoff(ra) + (pca ? x1 : x0));
-            pb[l] = pix(rowoff(rb) + (pca ? x0 : x1));
-        }
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int l = 0; l < 8; l++) {
             const int ra = l >= 4 ? l - 4 : 11 - l, rb = l >= 4 ? 11 - l : 12 + l;
             const int pca = (ra + S) & 1;
-            const float da = pa[l] * wv[ra];
-            const float db = pb[l] * wv[rb];
+            const float da = F.p[l] * wv[ra];
+            const float db = F.p[8 + l] * wv[rb];
             f[l] = fmaf(da, pca ? whc1 : whc0, db * (pca ? whc0 : whc1));
             fs[l] = fmaf(da, pca ? whs1 : whs0, db * (pca ? whs0 : whs1));
         }
@@ -449,12 +500,42 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827;
     // one half row (4 bins) of the 4 kernel quadrants at a time
     {
-#if !TP_KERN_EARLY
+#if TP_KERN_SHARE && !TP_KERN_EARLY
+        // Units u and u + 4 of a wavefront (lanes l and l + 32) are one
+        // camera and color of two neighbouring tiles.  When their kernel
+        // cells coincide (about half the time) only the lower unit loads the
+        // rows, and the upper one takes them across the wavefront halves
+        // (v_permlane32_swap: the upper half of the first result holds the
+        // lower half of the source).
+        f32x4 kpre[8];
+        {
+            const int kfi_lo = (int)__builtin_amdgcn_permlane32_swap((unsigned)F.kfi, (unsigned)F.kfi,
+                                                                      false, false)[0];
+            const bool share = __lane_id() >= 32 && kfi_lo == F.kfi;
+            gf32x4 *kp = gptr4(kern + li * 8);
+#pragma unroll
+            for (int i = 0; i < 8; i++) kpre[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (!share) {
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const unsigned v = __float_as_uint(kpre[i][c]);
+                    const float lo = __uint_as_float(__builtin_amdgcn_permlane32_swap(v, v, false, false)[0]);
+                    kpre[i][c] = share ? lo : kpre[i][c];
+                }
+        }
+#elif !TP_KERN_EARLY
         gf32x4 *kp = gptr4(kern + li * 8);
 #endif
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-#if TP_KERN_EARLY
+#if TP_KERN_EARLY || TP_KERN_SHARE
             const f32x4 K0 = kpre[h * 4], K1 = kpre[h * 4 + 1], K2 = kpre[h * 4 + 2], K3 = kpre[h * 4 + 3];
 #else
             const f32x4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
@@ -518,6 +599,31 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     }
 }
 
+// One unit (tile task_num, camera cam, color COLOR) up to the optional LPF:
+// unit_fetch then unit_compute.  pre: the unit's centre and CltExtra if the
+// caller loaded them ahead (else they are loaded here).
+template <int COLOR, int DCT = 0>
+__device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFold &lf, float *lds_c,
+                                             float *lds_s, int li, int cam, int task_num,
+                                             float (&q0)[8], float (&q1)[8], float (&q2)[8],
+                                             float (&q3)[8], const CvtMfma<DCT> &mt = {},
+                                             int ustride = 72, const UnitPre *pre = nullptr) {
+    UnitPre up;
+    if (pre) {
+        up = *pre;
+    } else {
+        unit_centre(a, cam, task_num, up);
+        unit_extra<COLOR>(a, cam, up);
+    }
+    UnitFetch F;
+    unit_fetch<COLOR>(a, lf, up, F);
+    // all sample loads are issued before the first use (the scheduling
+    // barrier keeps the compiler from pairing each load with its use, which
+    // serialises the HBM latencies when it is short of registers)
+    __builtin_amdgcn_sched_barrier(0);
+    unit_compute<COLOR, DCT>(a, lf, lds_c, lds_s, li, F, q0, q1, q2, q3, mt, ustride);
+}
+
 // Tile index of a task, or -1 for a task outside the tile grid (skipped).
 __device__ __forceinline__ int task_tile(const Geo &g, int txy) {
     const int tx = txy & 0xfff, ty = txy >> 16;
@@ -539,41 +645,47 @@ __device__ __forceinline__ void store_clt_rows(float *dst, int li, const float (
     store4(dst + 196, q3[4], q3[5], q3[6], q3[7]);
 }
 
-// One unit through to memory.  slice: the unit's 256-float LDS block (its
-// transposition staging, then its output block).
+// Output block of a unit, [q][l][k], :3600-3630; st: whether it is stored
+// (an empty task or one outside the tile grid is skipped).
 template <int COLOR>
-__device__ __forceinline__ void convert_store_unit(const ConvertArgs &a, const LaneFold &lf,
-                                                   float *slice, int li, int cam, int task_num,
-                                                   bool valid, const UnitPre *pre = nullptr) {
-    float q0[8], q1[8], q2[8], q3[8];
-    convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, {}, 256, pre);
-    // (12) store [q][l][k], :3600-3630; out-of-grid tiles are skipped
-    const int tile = task_tile(a.g, pre ? pre->txy : gtasks(a.tasks)[task_num].txy);
-    const bool st = valid && tile >= 0;
-    float *dst = a.l_clt + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
-#if TP_CONVERT_LDS_STORE
-    // The unit's 1 KB block goes through LDS so that store j writes float4
-    // j * 8 + li: every store instruction covers whole 128-byte lines of
-    // each unit (non-temporal stores of partial lines cost extra writes).
-    // The slice was last read before convert_unit's final wavefront fence.
-    {
-        const float *q[4] = {q0, q1, q2, q3};
+__device__ __forceinline__ float *unit_dst(const ConvertArgs &a, int txy, bool valid, bool &st) {
+    const int tile = task_tile(a.g, txy);
+    st = valid && tile >= 0;
+    return a.l_clt + ((size_t)max(tile, 0) * TP_NUM_COLORS + COLOR) * 256;
+}
+
+// The unit's 1 KB block goes through LDS so that store j writes float4
+// j * 8 + li: every store instruction covers whole 128-byte lines of each
+// unit (non-temporal stores of partial lines cost extra writes).  stage_block
+// writes the block into the unit's slice (last read before unit_compute's
+// final wavefront fence), store_block copies it to memory.
+__device__ __forceinline__ void stage_block(float *slice, int li, const float (&q0)[8],
+                                            const float (&q1)[8], const float (&q2)[8],
+                                            const float (&q3)[8]) {
+    const float *q[4] = {q0, q1, q2, q3};
 #pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
-            *(float4 *)(slice + qq * 64 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
-            *(float4 *)(slice + qq * 64 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
-        }
+    for (int qq = 0; qq < 4; qq++) {
+        *(float4 *)(slice + qq * 64 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
+        *(float4 *)(slice + qq * 64 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
     }
     wave_lds_sync();
+}
+// Write sink of the blocks that are not stored (empty tasks, tiles outside
+// the grid, slots past the end of the list): their stores go here instead
+// of being branched around, so every path issues the same memory
+// operations and the compiler's wait counts for the loads issued before
+// them do not have to assume the shorter path (which would make a wait for
+// those loads also wait for the stores).  Never read.
+__device__ __align__(256) float g_convert_sink[256];
+
+__device__ __forceinline__ void store_block(const float *slice, float *dst, int li, bool st) {
+    float *base = st ? dst : g_convert_sink;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const f32x4 v = *(const f32x4 *)(slice + (j * 8 + li) * 4);
-        if (st) store4(dst + (j * 8 + li) * 4, v[0], v[1], v[2], v[3]);
+        store4(base + (j * 8 + li) * 4, v[0], v[1], v[2], v[3]);
     }
     wave_lds_sync();  // the slice is the next unit's staging
-#else
-    if (st) store_clt_rows(dst, li, q0, q1, q2, q3);
-#endif
 }
 
 // Task of unit slot `unit` in tile pair p of the visiting order (clamped
@@ -590,14 +702,20 @@ __device__ __forceinline__ int pair_task(const ConvertArgs &a, int p, int unit, 
         task_num = (unit >> 2) ? t1 : t0;
         if (!valid) task_num = a.num_tiles - 1;
     } else {
-        task_num = valid ? swizzle_strip(s, a.g.TX, a.num_tiles) : a.num_tiles - 1;
+        task_num = !valid ? a.num_tiles - 1
+                 : TP_CONVERT_GROUP_W > 0
+                     ? swizzle_group(s, a.g.TX, a.num_tiles, TP_CONVERT_GROUP_H, TP_CONVERT_GROUP_W)
+                     : swizzle_strip(s, a.g.TX, a.num_tiles);
     }
     return task_num;
 }
 
-// The pairs [p0, p1) of one wavefront (one color), software-pipelined: while
-// pair p is converted, the CltExtra of pair p + 1 (whose centre arrived
-// during pair p - 1) and the centre of pair p + 2 are in flight.
+// The pairs [p0, p1) of one wavefront (one color), software-pipelined:
+// while pair p is converted, the CltExtra of pair p + 1 (whose centre
+// arrived during pair p - 1) and the centre of pair p + 2 are in flight, and
+// (TP_CONVERT_EARLY_FETCH) the Bayer samples of pair p + 1 are requested
+// before pair p's stores, so that waiting for them does not wait for the
+// stores.
 template <int COLOR>
 __device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFold &lf, float *slice,
                                              int li, int cam, int unit, int p0, int p1) {
@@ -613,6 +731,10 @@ __device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFol
     unit_centre(a, cam, t0, u0);
     unit_centre(a, cam, t1, u1);
     unit_extra<COLOR>(a, cam, u0);
+#if TP_CONVERT_EARLY_FETCH
+    UnitFetch f0;
+    unit_fetch<COLOR>(a, lf, u0, f0);
+#endif
     for (int p = p0; p < p1; p++) {
         bool v2;
         const int t2 = pair_task(a, p + 2, unit, v2);
@@ -621,8 +743,26 @@ __device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFol
         unit_centre(a, cam, t2, u2);
         unit_extra<COLOR>(a, cam, u1);
         __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of this pair's work
-        const bool valid = v0 && u0.task != 0;
-        convert_store_unit<COLOR>(a, lf, slice, li, cam, t0, valid, &u0);
+        float q0[8], q1[8], q2[8], q3[8];
+#if TP_CONVERT_EARLY_FETCH
+        unit_compute<COLOR>(a, lf, slice, slice + 72, li, f0, q0, q1, q2, q3, {}, 256);
+#else
+        convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, t0, q0, q1, q2, q3, {}, 256, &u0);
+#endif
+        bool st;
+        float *dst = unit_dst<COLOR>(a, u0.txy, v0 && u0.task != 0, st);
+#if TP_CONVERT_LDS_STORE
+        stage_block(slice, li, q0, q1, q2, q3);
+#endif
+#if TP_CONVERT_EARLY_FETCH
+        unit_fetch<COLOR>(a, lf, u1, f0);  // pair p + 1
+        __builtin_amdgcn_sched_barrier(0);  // its loads ahead of this pair's stores
+#endif
+#if TP_CONVERT_LDS_STORE
+        store_block(slice, dst, li, st);
+#else
+        if (st) store_clt_rows(dst, li, q0, q1, q2, q3);
+#endif
         u0 = u1;
         t0 = t1;
         v0 = v1;
@@ -654,6 +794,92 @@ void k_convert_correct_tiles(ConvertArgs a) {
         convert_loop<1>(la, lf, slice, li, cam, unit, p0, p1);
     else
         convert_loop<2>(la, lf, slice, li, cam, unit, p0, p1);
+}
+
+// All three colors of a unit in one wavefront (TP_CONVERT_RGB): the wavefront
+// converts R, B, then G of its 8 (tile, camera) units, so every wavefront
+// does the same work (G costs about twice R or B) and the pixel lines loaded
+// for R are still in L1 for B and G; the centre is loaded once per pair.
+template <int COLOR>
+__device__ __forceinline__ void rgb_unit(const ConvertArgs &a, const LaneFold &lf, float *slice, int li,
+                                         const UnitPre &u, bool valid) {
+    UnitFetch F;
+    unit_fetch<COLOR>(a, lf, u, F);
+    __builtin_amdgcn_sched_barrier(0);
+    float q0[8], q1[8], q2[8], q3[8];
+    unit_compute<COLOR>(a, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    bool st;
+    float *dst = unit_dst<COLOR>(a, u.txy, valid, st);
+    stage_block(slice, li, q0, q1, q2, q3);
+    store_block(slice, dst, li, st);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
+void k_convert_rgb(ConvertArgs a) {
+    __shared__ __align__(16) float lds[4][8][256];  // [wave][unit]: staging, then output block
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int unit = lane >> 3;
+    const int li = lane & 7;
+    const int cam = unit & 3;
+    const int npairs = (a.num_tiles + 1) >> 1;
+    const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+    const int p0 = w * a.pairs_per_block;
+    const int p1 = min(npairs, p0 + a.pairs_per_block);
+    if (p0 >= p1) return;
+    float *slice = &lds[wave][unit][0];
+    const LaneFold lf = lane_fold(li);
+    ConvertArgs la = a;
+    resolve_cam(la, cam);
+    bool v0;
+    int t0 = pair_task(la, p0, unit, v0);
+    UnitPre u0;
+    unit_centre(la, cam, t0, u0);
+    for (int p = p0; p < p1; p++) {
+        bool v1;
+        const int t1 = pair_task(la, p + 1, unit, v1);
+        v1 = v1 && p + 1 < p1;
+        UnitPre u1;
+        unit_centre(la, cam, t1, u1);  // the next pair's centre in flight
+        UnitPre ur = u0, ub = u0, ug = u0;
+        unit_extra<0>(la, cam, ur);
+        unit_extra<1>(la, cam, ub);
+        unit_extra<2>(la, cam, ug);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool valid = v0 && ur.task != 0;
+#if TP_CONVERT_EARLY_FETCH
+        // each color's samples requested before the previous color's stores
+        UnitFetch F;
+        float q0[8], q1[8], q2[8], q3[8];
+        bool st;
+        float *dst;
+        unit_fetch<0>(la, lf, ur, F);
+        __builtin_amdgcn_sched_barrier(0);
+        unit_compute<0>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+        dst = unit_dst<0>(la, ur.txy, valid, st);
+        stage_block(slice, li, q0, q1, q2, q3);
+        unit_fetch<1>(la, lf, ub, F);
+        __builtin_amdgcn_sched_barrier(0);
+        store_block(slice, dst, li, st);
+        unit_compute<1>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+        dst = unit_dst<1>(la, ub.txy, valid, st);
+        stage_block(slice, li, q0, q1, q2, q3);
+        unit_fetch<2>(la, lf, ug, F);
+        __builtin_amdgcn_sched_barrier(0);
+        store_block(slice, dst, li, st);
+        unit_compute<2>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+        dst = unit_dst<2>(la, ug.txy, valid, st);
+        stage_block(slice, li, q0, q1, q2, q3);
+        store_block(slice, dst, li, st);
+#else
+        rgb_unit<0>(la, lf, slice, li, ur, valid);
+        rgb_unit<1>(la, lf, slice, li, ub, valid);
+        rgb_unit<2>(la, lf, slice, li, ug, valid);
+#endif
+        u0 = u1;
+        t0 = t1;
+        v0 = v1;
+    }
 }
 
 }  // namespace tp

@@ -34,17 +34,35 @@
 #ifndef TP_KERN_EARLY
 #define TP_KERN_EARLY 0    // forward CLT: load the aberration kernel row before the pixels
 #endif
+#ifndef TP_CONVERT_RGB
+#define TP_CONVERT_RGB 1   // staged forward CLT: one wavefront converts all 3 colors of its units
+#endif
+#ifndef TP_CONVERT_RGB_PPW
+#define TP_CONVERT_RGB_PPW 2  // ... tile pairs per wavefront
+#endif
+#ifndef TP_KERN_SHARE
+#define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
+#endif
 #ifndef TP_STRIP_ROWS
 #define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
 #endif
 #ifndef TP_PANEL_W
 #define TP_PANEL_W 0       // forward CLT visiting order: panel width in tiles (0: full-width strips)
 #endif
+#ifndef TP_CONVERT_GROUP_W
+#define TP_CONVERT_GROUP_W 0  // staged forward CLT: group order width in tiles (0: column-pair strips)
+#endif
+#ifndef TP_CONVERT_GROUP_H
+#define TP_CONVERT_GROUP_H 4  // staged forward CLT: group order strip height in tile rows
+#endif
 #ifndef TP_IMG_NT
 #define TP_IMG_NT 0        // forward CLT: non-temporal pixel loads
 #endif
 #ifndef TP_CONVERT_LDS_STORE
 #define TP_CONVERT_LDS_STORE 1  // forward CLT output through LDS as whole 128-byte lines
+#endif
+#ifndef TP_CONVERT_EARLY_FETCH
+#define TP_CONVERT_EARLY_FETCH 1  // forward CLT: next unit's samples requested before this unit's stores
 #endif
 #ifndef TP_CORR_BLOCK_STORE
 #define TP_CORR_BLOCK_STORE 1  // correlation rows of radius 7 written as contiguous 8-row blocks
