@@ -33,7 +33,7 @@ RADIUS = 7
 # this benchmark, calibrated on a 1 GiB device copy (written by
 # tile_processor_gpu_amd.rocprof_summary; the matching .md gives the commands)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clt_corr_full_frame_traffic.json")
-KERNEL_SYMBOLS = {"convert_correct_tiles": "tp::k_convert_correct_tiles",
+KERNEL_SYMBOLS = {"convert_correct_tiles": "tp::k_convert_rgb",
                   "correlate2D": "void tp::k_correlate2D<3>"}
 
 
@@ -363,7 +363,7 @@ def main():
     ap.add_argument("--err-tiles", type=int, default=512,
                     help="tiles checked against the FP64 oracle after timing (0: skip)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="only the plain staged path (no fused / stream measurements)")
+                    help="only the plain staged path (no HBM ceilings, fused or stream measurements)")
     ap.add_argument("--stream-frames", type=int, default=64)
     ap.add_argument("--mode", choices=["frame", "copy", "fused"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
@@ -452,13 +452,15 @@ def main():
     if tr is not None:
         roofline["traffic"] = tr["bytes"]
         roofline["traffic_detail"] = tr
-    ceilings = hbm_ceilings()
-    log(f"hbm ceilings: {ceilings}")
-    # the dominant kernel's own read/write mix: convert reads about one byte
-    # per three written, correlate about one per two thirds
-    practical = ceilings["mix_1r3w_GBps" if dom == "convert_correct_tiles" else "copy_GBps"]
-    roofline["practical_ceiling_GBps"] = practical
-    roofline["frac_of_practical"] = round(d["GBps"] / practical, 4)
+    ceilings = None
+    if not args.no_extra:
+        ceilings = hbm_ceilings()
+        log(f"hbm ceilings: {ceilings}")
+        # the dominant kernel's own read/write mix: convert reads about one
+        # byte per three written, correlate about one per two thirds
+        practical = ceilings["mix_1r3w_GBps" if dom == "convert_correct_tiles" else "copy_GBps"]
+        roofline["practical_ceiling_GBps"] = practical
+        roofline["frac_of_practical"] = round(d["GBps"] / practical, 4)
     if tr is not None:
         roofline["traffic_GBps"] = round(tr["bytes"] / (d["ms"] * 1e-3) / 1e9, 1)
     frame_bytes = conv_b + corr_b
