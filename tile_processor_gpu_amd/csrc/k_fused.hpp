@@ -179,21 +179,28 @@ void k_fused(FusedArgs a) {
 
         for (int gs = s0; gs < s1; gs += GT) {
             // (1) forward CLT of the group into LDS
-            // one pass converts 2 tiles (8 units per wavefront); kept as a
-            // loop so that two passes' registers are not live together
+            // one job converts one color of 2 tiles (8 units per wavefront);
+            // kept as a loop so that two jobs' registers are not live
+            // together.  With GT = 4 (two tile pairs) and TP_FUSED_BALANCE
+            // the six jobs are spread so that every wavefront does about the
+            // same work (G costs about twice R or B): wavefronts 0 and 1 do
+            // G then R of pair 0 / pair 1, wavefront 2 does B of both pairs.
+            // Otherwise wavefront w converts color w of every pair.
 #pragma unroll 1
-            for (int pass = 0; pass < GT / 2; pass++) {
+            for (int job = 0; job < GT / 2; job++) {
+                constexpr bool BAL = TP_FUSED_BALANCE && GT == 4;
+                const int color = BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
+                const int pass = BAL ? (wave == 2 ? job : wave) : job;
                 const int tl = pass * 2 + (unit >> 2);
                 const int s = gs + tl;
                 bool valid = s < s1;
                 const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
                 valid = valid && gtasks(fr.tasks)[task_num].task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
-                const int color = wave;
                 float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * 256;  // units 768 floats apart
-                if (wave == 0)
+                if (color == 0)
                     convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
-                else if (wave == 1)
+                else if (color == 1)
                     convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
                 else
                     convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);

@@ -23,13 +23,16 @@
 #define TP_CONVERT_PPB 4   // tile pairs per workgroup
 #endif
 #ifndef TP_CORR_EPB
-#define TP_CORR_EPB 64     // correlation entries per workgroup (multiple of 32)
+#define TP_CORR_EPB 32     // correlation entries per workgroup (multiple of 32)
 #endif
 #ifndef TP_FUSED_WAVES
 #define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
 #endif
 #ifndef TP_FUSED_QWAVES
 #define TP_FUSED_QWAVES 3  // FP32 fused kernel: minimum waves per SIMD
+#endif
+#ifndef TP_FUSED_BALANCE
+#define TP_FUSED_BALANCE 1 // fused kernel (4 tiles per group): color jobs spread evenly over wavefronts
 #endif
 #ifndef TP_KERN_EARLY
 #define TP_KERN_EARLY 0    // forward CLT: load the aberration kernel row before the pixels
