@@ -33,7 +33,7 @@ RADIUS = 7
 # this benchmark, calibrated on a 1 GiB device copy (written by
 # tile_processor_gpu_amd.rocprof_summary; the matching .md gives the commands)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clt_corr_full_frame_traffic.json")
-KERNEL_SYMBOLS = {"convert_correct_tiles": "tp::k_convert_rgb",
+KERNEL_SYMBOLS = {"convert_correct_tiles": "tp::k_convert_cam",
                   "correlate2D": "void tp::k_correlate2D<3>"}
 
 

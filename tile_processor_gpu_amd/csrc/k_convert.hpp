@@ -814,6 +814,47 @@ __device__ __forceinline__ void rgb_unit(const ConvertArgs &a, const LaneFold &l
     store_block(slice, dst, li, st);
 }
 
+// R, B and G of the wavefront's 8 units (u0: their centres, task words
+// loaded; v0: whether the units' slots are real list entries).
+__device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold &lf, float *slice, int li,
+                                          int cam, const UnitPre &u0, bool v0) {
+    UnitPre ur = u0, ub = u0, ug = u0;
+    unit_extra<0>(la, cam, ur);
+    unit_extra<1>(la, cam, ub);
+    unit_extra<2>(la, cam, ug);
+    __builtin_amdgcn_sched_barrier(0);
+    const bool valid = v0 && ur.task != 0;
+#if TP_CONVERT_EARLY_FETCH
+    // each color's samples requested before the previous color's stores
+    UnitFetch F;
+    float q0[8], q1[8], q2[8], q3[8];
+    bool st;
+    float *dst;
+    unit_fetch<0>(la, lf, ur, F);
+    __builtin_amdgcn_sched_barrier(0);
+    unit_compute<0>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    dst = unit_dst<0>(la, ur.txy, valid, st);
+    stage_block(slice, li, q0, q1, q2, q3);
+    unit_fetch<1>(la, lf, ub, F);
+    __builtin_amdgcn_sched_barrier(0);
+    store_block(slice, dst, li, st);
+    unit_compute<1>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    dst = unit_dst<1>(la, ub.txy, valid, st);
+    stage_block(slice, li, q0, q1, q2, q3);
+    unit_fetch<2>(la, lf, ug, F);
+    __builtin_amdgcn_sched_barrier(0);
+    store_block(slice, dst, li, st);
+    unit_compute<2>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    dst = unit_dst<2>(la, ug.txy, valid, st);
+    stage_block(slice, li, q0, q1, q2, q3);
+    store_block(slice, dst, li, st);
+#else
+    rgb_unit<0>(la, lf, slice, li, ur, valid);
+    rgb_unit<1>(la, lf, slice, li, ub, valid);
+    rgb_unit<2>(la, lf, slice, li, ug, valid);
+#endif
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_rgb(ConvertArgs a) {
     __shared__ __align__(16) float lds[4][8][256];  // [wave][unit]: staging, then output block
@@ -841,41 +882,57 @@ void k_convert_rgb(ConvertArgs a) {
         v1 = v1 && p + 1 < p1;
         UnitPre u1;
         unit_centre(la, cam, t1, u1);  // the next pair's centre in flight
-        UnitPre ur = u0, ub = u0, ug = u0;
-        unit_extra<0>(la, cam, ur);
-        unit_extra<1>(la, cam, ub);
-        unit_extra<2>(la, cam, ug);
-        __builtin_amdgcn_sched_barrier(0);
-        const bool valid = v0 && ur.task != 0;
-#if TP_CONVERT_EARLY_FETCH
-        // each color's samples requested before the previous color's stores
-        UnitFetch F;
-        float q0[8], q1[8], q2[8], q3[8];
-        bool st;
-        float *dst;
-        unit_fetch<0>(la, lf, ur, F);
-        __builtin_amdgcn_sched_barrier(0);
-        unit_compute<0>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
-        dst = unit_dst<0>(la, ur.txy, valid, st);
-        stage_block(slice, li, q0, q1, q2, q3);
-        unit_fetch<1>(la, lf, ub, F);
-        __builtin_amdgcn_sched_barrier(0);
-        store_block(slice, dst, li, st);
-        unit_compute<1>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
-        dst = unit_dst<1>(la, ub.txy, valid, st);
-        stage_block(slice, li, q0, q1, q2, q3);
-        unit_fetch<2>(la, lf, ug, F);
-        __builtin_amdgcn_sched_barrier(0);
-        store_block(slice, dst, li, st);
-        unit_compute<2>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
-        dst = unit_dst<2>(la, ug.txy, valid, st);
-        stage_block(slice, li, q0, q1, q2, q3);
-        store_block(slice, dst, li, st);
-#else
-        rgb_unit<0>(la, lf, slice, li, ur, valid);
-        rgb_unit<1>(la, lf, slice, li, ub, valid);
-        rgb_unit<2>(la, lf, slice, li, ug, valid);
-#endif
+        rgb_units(la, lf, slice, li, cam, u0, v0);
+        u0 = u1;
+        t0 = t1;
+        v0 = v1;
+    }
+}
+
+
+// Camera per wavefront (TP_CONVERT_CAMWAVE): the 4 wavefronts of a
+// workgroup convert the 4 cameras of the same 8 tiles, a block of
+// TP_CAM_BLOCK_H x TP_CAM_BLOCK_W tiles in the group order.  A wavefront's 8
+// units are 8 neighbouring tiles of one camera, so units that share an
+// aberration kernel cell (2 x 2 tiles per cell) or image lines load the same
+// addresses in the same instruction, and the camera's pointers are
+// wavefront-uniform.
+__device__ __forceinline__ int block_task(const ConvertArgs &a, int b, int unit, bool &valid) {
+    const int s = b * 8 + unit;
+    valid = s < a.num_tiles;
+    return valid ? swizzle_group(s, a.g.TX, a.num_tiles, TP_CAM_BLOCK_H, TP_CAM_BLOCK_W) : a.num_tiles - 1;
+}
+
+__global__ __launch_bounds__(64 * TP_CAM_WG_WAVES) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
+void k_convert_cam(ConvertArgs a) {
+    static_assert(TP_CAM_BLOCK_H * TP_CAM_BLOCK_W == 8, "8 tiles per block");
+    static_assert(TP_CAM_WG_WAVES == 4 || TP_CAM_WG_WAVES == 1, "wavefronts per workgroup");
+    __shared__ __align__(16) float lds[TP_CAM_WG_WAVES][8][256];  // [wave][unit]: staging, then output block
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int unit = lane >> 3;
+    const int li = lane & 7;
+    // with one wavefront per workgroup, workgroup 4 k + c is camera c of block group k
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int cam = TP_CAM_WG_WAVES == 4 ? wave : (wg & 3);
+    const int nblk = (a.num_tiles + 7) >> 3;
+    const int b0 = (TP_CAM_WG_WAVES == 4 ? wg : wg >> 2) * a.pairs_per_block;
+    const int b1 = min(nblk, b0 + a.pairs_per_block);
+    float *slice = &lds[wave][unit][0];
+    const LaneFold lf = lane_fold(li);
+    ConvertArgs la = a;
+    resolve_cam(la, cam);
+    bool v0;
+    int t0 = block_task(la, b0, unit, v0);
+    UnitPre u0;
+    unit_centre(la, cam, t0, u0);
+    for (int b = b0; b < b1; b++) {
+        bool v1;
+        const int t1 = block_task(la, b + 1, unit, v1);
+        v1 = v1 && b + 1 < b1;
+        UnitPre u1;
+        unit_centre(la, cam, t1, u1);  // the next block's centre in flight
+        rgb_units(la, lf, slice, li, cam, u0, v0);
         u0 = u1;
         t0 = t1;
         v0 = v1;

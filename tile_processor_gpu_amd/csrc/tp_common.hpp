@@ -43,6 +43,19 @@
 #ifndef TP_CONVERT_RGB_PPW
 #define TP_CONVERT_RGB_PPW 2  // ... tile pairs per wavefront
 #endif
+#ifndef TP_CONVERT_CAMWAVE
+#define TP_CONVERT_CAMWAVE 1  // staged forward CLT: one camera per wavefront, 8 neighbouring tiles
+#endif
+#ifndef TP_CONVERT_CAM_BPW
+#define TP_CONVERT_CAM_BPW 1  // ... 8-tile blocks per workgroup
+#endif
+#ifndef TP_CAM_BLOCK_W
+#define TP_CAM_BLOCK_W 8      // ... block width in tiles (height 8 / width)
+#endif
+#define TP_CAM_BLOCK_H (8 / TP_CAM_BLOCK_W)
+#ifndef TP_CAM_WG_WAVES
+#define TP_CAM_WG_WAVES 4     // ... wavefronts per workgroup: 4 (the 4 cameras) or 1
+#endif
 #ifndef TP_KERN_SHARE
 #define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
 #endif
