@@ -173,9 +173,9 @@ __device__ __forceinline__ int pair_cam2(int pair) {
 }
 
 template <int COLORS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
+__global__ __launch_bounds__(64 * TP_CORR_WG_WAVES) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
 void k_correlate2D(CorrArgs a) {
-    __shared__ __align__(16) float lds[4][8][4][72];  // [wave][unit][quadrant][8x8+pad]
+    __shared__ __align__(16) float lds[TP_CORR_WG_WAVES][8][4][72];  // [wave][unit][quadrant][8x8+pad]
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -190,7 +190,7 @@ void k_correlate2D(CorrArgs a) {
     const bool dense = a.radius == 7 && a.stride == 225 &&
                        (((uintptr_t)a.corrs) & 15) == 0;
 
-    for (int eb = e0 + wave * 8; eb < e1; eb += 32) {
+    for (int eb = e0 + wave * 8; eb < e1; eb += 8 * TP_CORR_WG_WAVES) {
         int e = eb + unit;
         bool valid = e < e1;
         if (!valid) e = e1 - 1;

@@ -253,9 +253,11 @@ int tp_correlate2D(const tp_geometry *geom, float **gpu_clt, int colors, float s
     a.entries_per_block = TP_CORR_EPB;
     const int grid = (int)((num_corr_tiles + a.entries_per_block - 1) / a.entries_per_block);
     if (colors == 3)
-        hipLaunchKernelGGL(tp::k_correlate2D<3>, dim3(grid), dim3(256), 0, as_stream(stream), a);
+        hipLaunchKernelGGL(tp::k_correlate2D<3>, dim3(grid), dim3(64 * TP_CORR_WG_WAVES), 0,
+                           as_stream(stream), a);
     else
-        hipLaunchKernelGGL(tp::k_correlate2D<1>, dim3(grid), dim3(256), 0, as_stream(stream), a);
+        hipLaunchKernelGGL(tp::k_correlate2D<1>, dim3(grid), dim3(64 * TP_CORR_WG_WAVES), 0,
+                           as_stream(stream), a);
     return check_launch("correlate2D");
 }
 

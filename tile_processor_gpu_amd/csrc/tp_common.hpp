@@ -22,8 +22,11 @@
 #ifndef TP_CONVERT_PPB
 #define TP_CONVERT_PPB 4   // tile pairs per workgroup
 #endif
+#ifndef TP_CORR_WG_WAVES
+#define TP_CORR_WG_WAVES 3  // correlation: wavefronts per workgroup (8 entries each per pass)
+#endif
 #ifndef TP_CORR_EPB
-#define TP_CORR_EPB 32     // correlation entries per workgroup (multiple of 32)
+#define TP_CORR_EPB (8 * TP_CORR_WG_WAVES)  // correlation entries per workgroup (multiple of 8 x waves)
 #endif
 #ifndef TP_FUSED_WAVES
 #define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
