@@ -291,9 +291,32 @@ def measure_stages(cfg, fb, idx, reps):
     ms = median_ms(lambda: tp.corr_combine(g, fb.num_corr, fb.corr_idx, fb.corrs, fb.corr_stride,
                                            RADIUS, 0x3F, comb, 225), reps)
     out["combine_ms"] = round(ms, 4)
+    # algorithmic bytes: the pair tiles read once, the combined tiles written
+    comb_bytes = fb.num_corr * (225 * 4 + 4) + g.num_tiles * 225 * 4
+    out["combine_GBps"] = round(comb_bytes / (ms * 1e-3) / 1e9, 1)
     disp = torch.zeros((g.num_tiles, 2), device=dev)
     out["centroid_ms"] = round(median_ms(lambda: tp.corr_disparity(comb, 225, g.num_tiles, RADIUS, 1,
                                                                     disp), reps), 4)
+    # inverse CLT of the whole frame (4 cameras x 3 colors, all four tile
+    # parities) into (W + 8) x (H + 8) planes: one output-stationary launch,
+    # and the reference's 48 per-pass launches (read-modify-write) of the
+    # same planes
+    dstride = g.img_width + 8
+    planes = [torch.zeros(3 * (g.img_height + 8) * dstride, device=dev) for _ in range(4)]
+    p_planes = tp.ptr_array(planes)
+    ms = median_ms(lambda: tp.imclt_rbg_all(g, fb.p_clt, p_planes, dstride), reps)
+    imclt_bytes = 4 * g.num_tiles * 3 * 1024 + 4 * 3 * (g.img_height + 8) * dstride * 4
+    out["imclt_frame_ms"] = round(ms, 4)
+    out["imclt_frame_GBps"] = round(imclt_bytes / (ms * 1e-3) / 1e9, 1)
+
+    def imclt_passes():
+        for cam in range(4):
+            for color in range(3):
+                for v in range(2):
+                    for h in range(2):
+                        tp.imclt_rbg(g, fb.clt[cam], planes[cam], color, v, h, dstride)
+    out["imclt_48_launches_ms"] = round(median_ms(imclt_passes, max(3, reps // 4), warmup=1), 4)
+    del planes
     # texture output
     t = np.ascontiguousarray(cfg["tasks"])
     tasks0 = torch.from_numpy(t.view(np.uint8).copy()).to(dev)

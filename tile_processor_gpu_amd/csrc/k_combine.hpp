@@ -36,39 +36,82 @@ struct CombineArgs {
     size_t out_stride;
 };
 
-// a workgroup scans 256 consecutive list entries; each group of entries of
-// one tile that starts there is combined by one wavefront (lanes = outputs)
+// A workgroup scans 256 consecutive list entries; each group of entries of
+// one tile that starts there is combined by one wavefront.  The entry words
+// (the workgroup's 256 and the 15 after them, which a group starting near
+// the end may reach) are kept in LDS, so finding a group's end and its pairs
+// costs no global loads.  The group's pair tiles (consecutive rows of the
+// correlation output) are copied into LDS with coalesced loads, 6 entries at
+// a time (all of a chunk's loads issued before any is used), then each lane
+// sums its outputs (lane + 64 t) over the mapped pair tiles.
 __global__ __launch_bounds__(256) void k_corr_combine(CombineArgs a) {
+    constexpr int CHUNK = 6;   // entries staged at a time (the 6 pairs of a full task)
+    constexpr int MAXG = 16;   // entries of one tile considered
     __shared__ int heads[256];
+    __shared__ int ent[256 + MAXG - 1];
     __shared__ int nheads;
+    __shared__ float stage[4][CHUNK * 225];
     const int e0 = blockIdx.x * 256;
-    const int e = e0 + threadIdx.x;
+    const int nent = min(256 + MAXG - 1, a.num_corr - e0);  // entry words held
     if (threadIdx.x == 0) nheads = 0;
+    for (int j = threadIdx.x; j < nent; j += 256) ent[j] = a.idx[e0 + j];
+    const int prev = (e0 > 0 && threadIdx.x == 0) ? a.idx[e0 - 1] >> TP_CORR_NTILE_SHIFT : 0;
     __syncthreads();
+    const int e = e0 + threadIdx.x;
     if (e < a.num_corr) {
-        const int t = a.idx[e] >> TP_CORR_NTILE_SHIFT;
-        if (e == 0 || (a.idx[e - 1] >> TP_CORR_NTILE_SHIFT) != t) heads[atomicAdd(&nheads, 1)] = e;
+        const int t = ent[threadIdx.x] >> TP_CORR_NTILE_SHIFT;
+        const int tp = threadIdx.x ? ent[threadIdx.x - 1] >> TP_CORR_NTILE_SHIFT : prev;
+        if (e == 0 || tp != t) heads[atomicAdd(&nheads, 1)] = threadIdx.x;
     }
     __syncthreads();
     const int nh = nheads;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = a.radius, n = 2 * r + 1;
+    const int r = a.radius, n = 2 * r + 1, nn = n * n;
+    float *L = stage[wave];
     for (int h = wave; h < nh; h += 4) {
-        const int g0 = heads[h];
-        const int tile = a.idx[g0] >> TP_CORR_NTILE_SHIFT;
+        const int j0 = heads[h];  // group start, relative to e0
+        const int tile = ent[j0] >> TP_CORR_NTILE_SHIFT;
         if (tile < 0 || tile >= a.ntiles) continue;
-        int ge = g0;
-        while (ge + 1 < a.num_corr && ge + 1 < g0 + 16 && (a.idx[ge + 1] >> TP_CORR_NTILE_SHIFT) == tile) ge++;
-        for (int o = lane; o < n * n; o += 64) {
-            const int cy = o / n - r, cx = o % n - r;
-            float s = 0.f;
-            for (int k = g0; k <= ge; k++) {
-                const int pair = a.idx[k] & ((1 << TP_CORR_NTILE_SHIFT) - 1);
-                if (pair >= TP_NUM_PAIRS || !((a.pair_mask >> pair) & 1)) continue;
-                int sy, sx;
-                if (pair_src(pair, r, cy, cx, sy, sx)) s += a.corrs[(size_t)k * a.stride + (sy + r) * n + sx + r];
+        int je = j0;
+        while (je + 1 < nent && je + 1 < j0 + MAXG && (ent[je + 1] >> TP_CORR_NTILE_SHIFT) == tile) je++;
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int c0 = j0; c0 <= je; c0 += CHUNK) {
+            const int ne = min(CHUNK, je + 1 - c0);
+            // copy rows e0 + c0 .. (nn floats each) into L[k][nn]
+            if (a.stride == (size_t)nn) {  // contiguous rows: one block
+                // all loads issued before the first LDS store (addresses
+                // clamped into the block, so none is conditional)
+                constexpr int PER_LANE = (CHUNK * 225 + 63) / 64;
+                gfloat *src = gptr(a.corrs + (size_t)(e0 + c0) * nn);
+                const int cnt = ne * nn;
+                float v[PER_LANE];
+#pragma unroll
+                for (int i = 0; i < PER_LANE; i++) v[i] = src[min(lane + 64 * i, cnt - 1)];
+#pragma unroll
+                for (int i = 0; i < PER_LANE; i++)
+                    if (lane + 64 * i < cnt) L[lane + 64 * i] = v[i];
+            } else {
+                for (int k = 0; k < ne; k++)
+                    for (int o = lane; o < nn; o += 64) L[k * nn + o] = a.corrs[(size_t)(e0 + c0 + k) * a.stride + o];
             }
-            a.out[(size_t)tile * a.out_stride + o] = s;
+            wave_lds_sync();
+            for (int k = 0; k < ne; k++) {
+                const int pair = ent[c0 + k] & ((1 << TP_CORR_NTILE_SHIFT) - 1);
+                if (pair >= TP_NUM_PAIRS || !((a.pair_mask >> pair) & 1)) continue;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int o = lane + 64 * t;
+                    const int cy = o / n - r, cx = o % n - r;
+                    int sy, sx;
+                    if (o < nn && pair_src(pair, r, cy, cx, sy, sx)) s[t] += L[k * nn + (sy + r) * n + sx + r];
+                }
+            }
+            wave_lds_sync();  // L is rewritten by the next chunk
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int o = lane + 64 * t;
+            if (o < nn) a.out[(size_t)tile * a.out_stride + o] = s[t];
         }
     }
 }
