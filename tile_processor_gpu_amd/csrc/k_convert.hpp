@@ -624,6 +624,13 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     unit_compute<COLOR, DCT>(a, lf, lds_c, lds_s, li, F, q0, q1, q2, q3, mt, ustride);
 }
 
+// LDS slice of one unit in the staged kernels: its 8 x 8 transposition
+// staging, then its 1 KB output block.  TP_SLICE_PAD floats of padding
+// would put the 8 units of a wavefront on different LDS banks (at a
+// 256-float stride the column reads of the transposition hit the same banks
+// from all units); measured without effect, so off.
+constexpr int kSlice = 256 + TP_SLICE_PAD;
+
 // Tile index of a task, or -1 for a task outside the tile grid (skipped).
 __device__ __forceinline__ int task_tile(const Geo &g, int txy) {
     const int tx = txy & 0xfff, ty = txy >> 16;
@@ -745,9 +752,9 @@ __device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFol
         __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of this pair's work
         float q0[8], q1[8], q2[8], q3[8];
 #if TP_CONVERT_EARLY_FETCH
-        unit_compute<COLOR>(a, lf, slice, slice + 72, li, f0, q0, q1, q2, q3, {}, 256);
+        unit_compute<COLOR>(a, lf, slice, slice + 72, li, f0, q0, q1, q2, q3, {}, kSlice);
 #else
-        convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, t0, q0, q1, q2, q3, {}, 256, &u0);
+        convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, t0, q0, q1, q2, q3, {}, kSlice, &u0);
 #endif
         bool st;
         float *dst = unit_dst<COLOR>(a, u0.txy, v0 && u0.task != 0, st);
@@ -774,7 +781,7 @@ __device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFol
 
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_correct_tiles(ConvertArgs a) {
-    __shared__ __align__(16) float lds[3][8][256];  // [wave][unit]: staging, then output block
+    __shared__ __align__(16) float lds[3][8][kSlice];  // [wave][unit]: staging, then output block
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -807,7 +814,7 @@ __device__ __forceinline__ void rgb_unit(const ConvertArgs &a, const LaneFold &l
     unit_fetch<COLOR>(a, lf, u, F);
     __builtin_amdgcn_sched_barrier(0);
     float q0[8], q1[8], q2[8], q3[8];
-    unit_compute<COLOR>(a, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    unit_compute<COLOR>(a, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, kSlice);
     bool st;
     float *dst = unit_dst<COLOR>(a, u.txy, valid, st);
     stage_block(slice, li, q0, q1, q2, q3);
@@ -832,19 +839,19 @@ __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold 
     float *dst;
     unit_fetch<0>(la, lf, ur, F);
     __builtin_amdgcn_sched_barrier(0);
-    unit_compute<0>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    unit_compute<0>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, kSlice);
     dst = unit_dst<0>(la, ur.txy, valid, st);
     stage_block(slice, li, q0, q1, q2, q3);
     unit_fetch<1>(la, lf, ub, F);
     __builtin_amdgcn_sched_barrier(0);
     store_block(slice, dst, li, st);
-    unit_compute<1>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    unit_compute<1>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, kSlice);
     dst = unit_dst<1>(la, ub.txy, valid, st);
     stage_block(slice, li, q0, q1, q2, q3);
     unit_fetch<2>(la, lf, ug, F);
     __builtin_amdgcn_sched_barrier(0);
     store_block(slice, dst, li, st);
-    unit_compute<2>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, 256);
+    unit_compute<2>(la, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, kSlice);
     dst = unit_dst<2>(la, ug.txy, valid, st);
     stage_block(slice, li, q0, q1, q2, q3);
     store_block(slice, dst, li, st);
@@ -857,7 +864,7 @@ __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold 
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_rgb(ConvertArgs a) {
-    __shared__ __align__(16) float lds[4][8][256];  // [wave][unit]: staging, then output block
+    __shared__ __align__(16) float lds[4][8][kSlice];  // [wave][unit]: staging, then output block
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -907,7 +914,7 @@ __global__ __launch_bounds__(64 * TP_CAM_WG_WAVES) __attribute__((amdgpu_waves_p
 void k_convert_cam(ConvertArgs a) {
     static_assert(TP_CAM_BLOCK_H * TP_CAM_BLOCK_W == 8, "8 tiles per block");
     static_assert(TP_CAM_WG_WAVES == 4 || TP_CAM_WG_WAVES == 1, "wavefronts per workgroup");
-    __shared__ __align__(16) float lds[TP_CAM_WG_WAVES][8][256];  // [wave][unit]: staging, then output block
+    __shared__ __align__(16) float lds[TP_CAM_WG_WAVES][8][kSlice];  // [wave][unit]: staging, then output block
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;

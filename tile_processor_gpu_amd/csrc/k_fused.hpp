@@ -89,7 +89,11 @@ void k_fused(FusedArgs a) {
     constexpr int NSLOT = 6 * GT;
     // [tile][camera][color][4 x 8 x 8]; a unit's slice also serves as its
     // transposition staging (2 x 72 floats) before its CLT is written there
-    __shared__ __align__(16) float clt_lds[GT * 4 * 3 * 256];
+    // each (tile, camera, color) block takes FB floats; TP_FUSED_PAD would
+    // put the 8 units of a wavefront on different LDS banks (measured
+    // without effect, so 0)
+    constexpr int FB = 256 + TP_FUSED_PAD;
+    __shared__ __align__(16) float clt_lds[GT * 4 * 3 * FB];
     __shared__ int s_item, s_frame;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -197,13 +201,13 @@ void k_fused(FusedArgs a) {
                 const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
                 valid = valid && gtasks(fr.tasks)[task_num].task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
-                float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * 256;  // units 768 floats apart
+                float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * FB;  // units 3 FB floats apart
                 if (color == 0)
-                    convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                    convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
                 else if (color == 1)
-                    convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                    convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
                 else
-                    convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 768);
+                    convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
                 float *d = slice + li * 8;
 #pragma unroll
                 for (int qq = 0; qq < 4; qq++) {
@@ -235,10 +239,10 @@ void k_fused(FusedArgs a) {
                     if (((tk->task >> (TP_TASK_CORR_BITS + pair)) & 1) && task_tile(g, tk->txy) >= 0)
                         row = 6 * task_num + pair;
                 }
-                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * 256 + li * 8;
-                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * 256 + li * 8;
+                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * FB + li * 8;
+                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * FB + li * 8;
                 auto load = [&](int color, int which, int qq, int h) -> f32x4 {
-                    return ((const f32x4 *)((which ? tb : ta) + color * 256))[qq * 16 + h];
+                    return ((const f32x4 *)((which ? tb : ta) + color * FB))[qq * 16 + h];
                 };
                 corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
             }

@@ -59,6 +59,12 @@
 #ifndef TP_CAM_WG_WAVES
 #define TP_CAM_WG_WAVES 4     // ... wavefronts per workgroup: 4 (the 4 cameras) or 1
 #endif
+#ifndef TP_SLICE_PAD
+#define TP_SLICE_PAD 0        // staged forward CLT: floats of padding per unit slice in LDS
+#endif
+#ifndef TP_FUSED_PAD
+#define TP_FUSED_PAD 0        // fused kernel: floats of padding per (tile, camera, color) block in LDS
+#endif
 #ifndef TP_KERN_SHARE
 #define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
 #endif
