@@ -28,27 +28,32 @@ __device__ __forceinline__ float hw2_sel(int i) {
 }
 
 // Steps 2-4 on one unit; q* in: row li, out: column li of each quadrant.
+// L: the unit's staging, NQ x 72 floats; NQ quadrants are transposed at a
+// time (NQ = 1 keeps the LDS footprint of a workgroup small).
+template <int NQ = 4>
 __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], float (&q2)[8],
                                                 float (&q3)[8], float *L, int li) {
+    static_assert(NQ == 1 || NQ == 2 || NQ == 4, "quadrants per transposition");
     dtt8<0>(q0);
     dtt8<1>(q1);
     dtt8<0>(q2);
     dtt8<1>(q3);
     float *q[4] = {q0, q1, q2, q3};
 #pragma unroll
-    for (int qq = 0; qq < 4; qq++) {
-        *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
-        *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
-    }
-    wave_lds_sync();
+    for (int q0i = 0; q0i < 4; q0i += NQ) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        q0[i] = L[0 * 72 + i * 8 + li];
-        q1[i] = L[1 * 72 + i * 8 + li];
-        q2[i] = L[2 * 72 + i * 8 + li];
-        q3[i] = L[3 * 72 + i * 8 + li];
+        for (int j = 0; j < NQ; j++) {
+            const float *v = q[q0i + j];
+            *(float4 *)(L + j * 72 + li * 8) = make_float4(v[0], v[1], v[2], v[3]);
+            *(float4 *)(L + j * 72 + li * 8 + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < NQ; j++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) q[q0i + j][i] = L[j * 72 + i * 8 + li];
+        wave_lds_sync();
     }
-    wave_lds_sync();
     dtt8<0>(q0);
     dtt8<0>(q1);
     dtt8<1>(q2);
@@ -116,10 +121,11 @@ __global__ __launch_bounds__(64) void k_imclt_pass(Geo g, const float *clt, floa
 }
 
 // Whole frame, output-stationary.  grid = (64x64 blocks, color, camera).
-__global__ __launch_bounds__(256) void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_IMCLT_WAVES)))
+void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
                                                      size_t dstride) {
     __shared__ __align__(16) float acc[64][65];
-    __shared__ __align__(16) float lds[4][8][4][72];
+    __shared__ __align__(16) float lds[4][8][TP_IMCLT_NQ][72];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int unit = lane >> 3, li = lane & 7;
     const int PW = g.W + 8, PH = g.H + 8;
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(256) void k_imclt_frame(Geo g, const float *const *
             float q0[8], q1[8], q2[8], q3[8];
             const int tyc = valid ? ty : 0, txc = valid ? tx : 0;
             load_rows(cl + ((size_t)(tyc * g.TX + txc) * TP_NUM_COLORS + color) * 256, li, q0, q1, q2, q3);
-            imclt_transform(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
+            imclt_transform<TP_IMCLT_NQ>(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
             if (valid) {
                 const int oy = ty * 8 - Y0, ox = tx * 8 - X0;
                 imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {

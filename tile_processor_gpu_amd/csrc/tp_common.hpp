@@ -65,6 +65,12 @@
 #ifndef TP_FUSED_PAD
 #define TP_FUSED_PAD 0        // fused kernel: floats of padding per (tile, camera, color) block in LDS
 #endif
+#ifndef TP_IMCLT_NQ
+#define TP_IMCLT_NQ 1         // whole-frame inverse CLT: quadrants per LDS transposition (1, 2, 4)
+#endif
+#ifndef TP_IMCLT_WAVES
+#define TP_IMCLT_WAVES 5      // whole-frame inverse CLT: minimum waves per SIMD
+#endif
 #ifndef TP_KERN_SHARE
 #define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
 #endif
