@@ -83,7 +83,8 @@ constexpr int fused_min_waves() {
 }
 
 template <int GT, bool QUEUE, int COLORS, int DCT = 0>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(fused_min_waves<GT, QUEUE, COLORS, DCT>())))
+__global__ __launch_bounds__(64 * TP_FUSED_WG_WAVES)
+__attribute__((amdgpu_waves_per_eu(fused_min_waves<GT, QUEUE, COLORS, DCT>())))
 void k_fused(FusedArgs a) {
     static_assert(GT == 2 || GT == 4, "tiles per group");
     constexpr int NSLOT = 6 * GT;
@@ -185,16 +186,22 @@ void k_fused(FusedArgs a) {
             // (1) forward CLT of the group into LDS
             // one job converts one color of 2 tiles (8 units per wavefront);
             // kept as a loop so that two jobs' registers are not live
-            // together.  With GT = 4 (two tile pairs) and TP_FUSED_BALANCE
-            // the six jobs are spread so that every wavefront does about the
-            // same work (G costs about twice R or B): wavefronts 0 and 1 do
-            // G then R of pair 0 / pair 1, wavefront 2 does B of both pairs.
-            // Otherwise wavefront w converts color w of every pair.
+            // together.  G costs about twice R or B.  With 4 wavefronts and
+            // GT = 4 (two tile pairs): wavefronts 0 and 1 do G of pair 0 /
+            // pair 1, wavefront 2 does R and wavefront 3 B of both pairs.
+            // With 3 wavefronts, GT = 4 and TP_FUSED_BALANCE: wavefronts 0
+            // and 1 do G then R of pair 0 / pair 1, wavefront 2 does B of
+            // both pairs.  Otherwise wavefront w converts color w of every
+            // pair (with 4 wavefronts and GT = 2, wavefront 3 rests).
+            constexpr int NW = TP_FUSED_WG_WAVES;
+            constexpr bool BAL = TP_FUSED_BALANCE && GT == 4 && NW == 3;
+            constexpr bool W4 = NW == 4 && GT == 4;
+            const int njobs = W4 ? (wave < 2 ? 1 : 2) : (NW == 4 && wave == 3) ? 0 : GT / 2;
 #pragma unroll 1
-            for (int job = 0; job < GT / 2; job++) {
-                constexpr bool BAL = TP_FUSED_BALANCE && GT == 4;
-                const int color = BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
-                const int pass = BAL ? (wave == 2 ? job : wave) : job;
+            for (int job = 0; job < njobs; job++) {
+                const int color = W4 ? (wave < 2 ? 2 : wave - 2)
+                                : BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
+                const int pass = W4 ? (wave < 2 ? wave : job) : BAL ? (wave == 2 ? job : wave) : job;
                 const int tl = pass * 2 + (unit >> 2);
                 const int s = gs + tl;
                 bool valid = s < s1;

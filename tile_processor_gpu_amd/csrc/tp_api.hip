@@ -107,7 +107,7 @@ tp::FusedArgs fused_args(const tp_geometry *geom, size_t dstride, int lpf_mask, 
 
 template <bool Q, int DCT>
 void launch_fused_d(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
-    const dim3 gd(grid), bd(192);
+    const dim3 gd(grid), bd(64 * TP_FUSED_WG_WAVES);
     if (c.gt == 2) {
         if (colors == 3) hipLaunchKernelGGL((tp::k_fused<2, Q, 3, DCT>), gd, bd, 0, st, a);
         else hipLaunchKernelGGL((tp::k_fused<2, Q, 1, DCT>), gd, bd, 0, st, a);

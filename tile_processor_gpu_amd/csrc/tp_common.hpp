@@ -34,6 +34,9 @@
 #ifndef TP_FUSED_QWAVES
 #define TP_FUSED_QWAVES 3  // FP32 fused kernel: minimum waves per SIMD
 #endif
+#ifndef TP_FUSED_WG_WAVES
+#define TP_FUSED_WG_WAVES 4   // fused kernel: wavefronts per workgroup (3 or 4)
+#endif
 #ifndef TP_FUSED_BALANCE
 #define TP_FUSED_BALANCE 1 // fused kernel (4 tiles per group): color jobs spread evenly over wavefronts
 #endif
