@@ -259,6 +259,17 @@ def measure_stream(nframes, reps, seed=4):
             fb.correlate(3, SCALES, FAT_ZERO)
     staged_ms = median_ms(staged, max(3, reps // 4), warmup=1)
     run.release()
+    # the same stream with the forward DCT on the matrix cores (FP16 hi + lo
+    # split), autotuned and graph-replayed; deviation from the FP32 stream
+    ref = [fb.corrs[:fb.num_corr].clone() for fb in fbs]
+    cfg16 = autotune.tune_stream(fbs, 3, SCALES, FAT_ZERO, cache_path=None, dct=1)
+    run16 = tp.StreamRun(fbs, 3, SCALES, FAT_ZERO, config=cfg16)
+    run16.capture()
+    ms16 = median_ms(run16.replay, reps)
+    run16.replay()
+    torch.cuda.synchronize()
+    dev16 = max(float((fb.corrs[:fb.num_corr] - r).abs().max() / r.abs().max()) for fb, r in zip(fbs, ref))
+    run16.release()
     return {"frames": nframes, "tiles": int(tiles),
             "active_fraction": [round(min(f["active"] for f in frames), 3),
                                 round(max(f["active"] for f in frames), 3)],
@@ -269,7 +280,10 @@ def measure_stream(nframes, reps, seed=4):
                            "service_p99": round(float(np.percentile(service, 99)), 4),
                            "completion_p50": round(float(np.percentile(done, 50)), 4),
                            "completion_p99": round(float(np.percentile(done, 99)), 4)},
-            "config": cfg, "tuned_over": len(report), "graph": True}
+            "config": cfg, "tuned_over": len(report), "graph": True,
+            "mfma_fp16x2": {"ms_per_batch": round(ms16, 4), "frames_per_s": round(nframes * 1e3 / ms16, 1),
+                            "speedup_vs_staged": round(staged_ms / ms16, 3), "config": cfg16,
+                            "max_rel_dev_vs_fp32_stream": dev16}}
 
 
 def measure_stages(cfg, fb, idx, reps):

@@ -267,3 +267,48 @@ def test_combine_and_disparity_match_oracle(small, orc):
     same_peak = np.abs(d[:, 1] - d_ref[:, 1]) <= 1e-5 * np.abs(d_ref[:, 1]).max()
     assert same_peak.all()
     assert np.abs(d[:, 0] - d_ref[:, 0]).max() < 1e-3
+
+
+def test_combine_irregular_list(small, orc):
+    """Combine over a list that is not the full tile-major one: random pair
+    subsets, tiles with duplicated entries (more than the 6 rows staged at a
+    time), tiles with no entry, radius 5 at a padded row stride.  Expected
+    output: per tile, the sum in list order of each entry's pair tile mapped
+    by the oracle's combination (one entry at a time)."""
+    g = small["g"]
+    rng = np.random.default_rng(21)
+    entries = []
+    for t in range(g.num_tiles):
+        u = rng.random()
+        if u < 0.1:
+            continue                                        # no entry
+        if u < 0.2:
+            pairs = list(range(6)) + list(range(4))         # 10 entries
+        else:
+            pairs = sorted(rng.choice(6, size=rng.integers(1, 7), replace=False).tolist())
+        entries += [(t << 8) | p for p in pairs]
+    idx = np.array(entries, np.int32)
+    r, stride = 5, 130
+    n = 2 * r + 1
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=r, max_corr=len(idx), corr_stride=stride)
+    for cam in range(4):
+        fb.clt[cam].copy_(torch.from_numpy(small["clt64"][cam].astype(np.float32)))
+    fb.upload(corr_idx=idx)
+    fb.correlate(3, SCALES, FAT_ZERO)
+    out = torch.full((g.num_tiles, n * n), -5.0, device="cuda:0")
+    tp.corr_combine(g, len(idx), fb.corr_idx, fb.corrs, stride, r, 0x3F, out, n * n)
+    torch.cuda.synchronize()
+    rows = fb.corrs[:len(idx)].cpu().numpy().astype(np.float64)
+    got = out.cpu().numpy()
+    ref = np.full((g.num_tiles, n * n), -5.0)
+    for k, e in enumerate(idx):
+        t, p = int(e) >> 8, int(e) & 0xFF
+        six = np.zeros((6, n * n))
+        six[p] = rows[k, :n * n]
+        mapped = orc.corr_combine(six, r, pair_mask=1 << p)[0]
+        first = k == 0 or (int(idx[k - 1]) >> 8) != t    # the list is grouped by tile
+        ref[t] = mapped if first else ref[t] + mapped
+    has = np.zeros(g.num_tiles, bool)
+    has[idx >> 8] = True
+    assert rel_err(got[has], ref[has]) < TOL
+    assert (got[~has] == -5.0).all()

@@ -78,16 +78,19 @@ def _store(path, cache):
 
 
 def tune_stream(frames, colors=3, scales=(0.25, 0.25, 0.5), fat_zero=0.05, reps=5, warmup=2,
-                cache_path=DEFAULT_CACHE, retune=False, report=None):
+                cache_path=DEFAULT_CACHE, retune=False, report=None, dct=0):
     """Fastest tp_fused_config for one persistent launch over `frames`
-    (a list of tp.FrameBuffers).  Writes the frames' correlation buffers.
-    report: optional list that receives (config, median ms) per candidate."""
-    key = _key("stream", frames)
+    (a list of tp.FrameBuffers), with the forward-DCT method dct (0 FP32,
+    1 FP16 hi + lo, 2 BF16 hi + lo on the matrix cores).  Writes the frames'
+    correlation buffers.  report: optional list that receives (config,
+    median ms) per candidate."""
+    key = _key(f"stream|dct={dct}", frames)
     cache = _load(cache_path) if cache_path else {}
     if not retune and key in cache:
         return dict(cache[key]["config"])
     best, best_ms = None, float("inf")
     for c in candidates(stream=True):
+        c = dict(c, dct=dct)
         run = tp.StreamRun(frames, colors, scales, fat_zero, config=c)
         ms = _median_ms(run.launch, reps, warmup)
         if report is not None:
