@@ -190,6 +190,11 @@ void k_correlate2D(CorrArgs a) {
     const bool dense = a.radius == 7 && a.stride == 225 &&
                        (((uintptr_t)a.corrs) & 15) == 0;
 
+    // the 4 cameras' CLT pointers, loaded once (uniform, scalar loads); a
+    // lane picks its pair's two by selection rather than by a dependent
+    // per-lane load between the list entry and the CLT rows
+    const float *cp0 = gelem(a.clt, 0), *cp1 = gelem(a.clt, 1), *cp2 = gelem(a.clt, 2), *cp3 = gelem(a.clt, 3);
+    auto cam_ptr = [&](int c) { return c == 0 ? cp0 : c == 1 ? cp1 : c == 2 ? cp2 : cp3; };
     for (int eb = e0 + wave * 8; eb < e1; eb += 8 * TP_CORR_WG_WAVES) {
         int e = eb + unit;
         bool valid = e < e1;
@@ -203,7 +208,7 @@ void k_correlate2D(CorrArgs a) {
             tile = 0;
         }
         const size_t off = (size_t)tile * TP_NUM_COLORS * 256 + li * 8;
-        const float *ca = a.clt[pair_cam1(pair)] + off, *cb = a.clt[pair_cam2(pair)] + off;
+        const float *ca = cam_ptr(pair_cam1(pair)) + off, *cb = cam_ptr(pair_cam2(pair)) + off;
         auto load = [&](int color, int which, int qq, int h) -> f32x4 {
             return gptr4((which ? cb : ca) + color * 256)[qq * 16 + h];
         };
