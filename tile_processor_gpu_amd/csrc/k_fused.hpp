@@ -182,6 +182,19 @@ void k_fused(FusedArgs a) {
         const int s0 = item * a.tiles_per_item;  // item: index within the frame
         const int s1 = min(fr.num_tiles, s0 + a.tiles_per_item);
 
+        // This wavefront's jobs (color, tile pair) in every group of GT tiles:
+        // see (1) below.
+        constexpr int NW = TP_FUSED_WG_WAVES;
+        constexpr bool BAL = TP_FUSED_BALANCE && GT == 4 && NW == 3;
+        constexpr bool W4 = NW == 4 && GT == 4;
+        const int njobs = W4 ? (wave < 2 ? 1 : 2) : (NW == 4 && wave == 3) ? 0 : GT / 2;
+        auto job_color = [&](int job) {
+            return W4 ? (wave < 2 ? 2 : wave - 2) : BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
+        };
+        auto job_pass = [&](int job) {
+            return W4 ? (wave < 2 ? wave : job) : BAL ? (wave == 2 ? job : wave) : job;
+        };
+
         for (int gs = s0; gs < s1; gs += GT) {
             // (1) forward CLT of the group into LDS
             // one job converts one color of 2 tiles (8 units per wavefront);
@@ -193,28 +206,30 @@ void k_fused(FusedArgs a) {
             // and 1 do G then R of pair 0 / pair 1, wavefront 2 does B of
             // both pairs.  Otherwise wavefront w converts color w of every
             // pair (with 4 wavefronts and GT = 2, wavefront 3 rests).
-            constexpr int NW = TP_FUSED_WG_WAVES;
-            constexpr bool BAL = TP_FUSED_BALANCE && GT == 4 && NW == 3;
-            constexpr bool W4 = NW == 4 && GT == 4;
-            const int njobs = W4 ? (wave < 2 ? 1 : 2) : (NW == 4 && wave == 3) ? 0 : GT / 2;
 #pragma unroll 1
             for (int job = 0; job < njobs; job++) {
-                const int color = W4 ? (wave < 2 ? 2 : wave - 2)
-                                : BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
-                const int pass = W4 ? (wave < 2 ? wave : job) : BAL ? (wave == 2 ? job : wave) : job;
+                const int color = job_color(job);
+                const int pass = job_pass(job);
                 const int tl = pass * 2 + (unit >> 2);
-                const int s = gs + tl;
-                bool valid = s < s1;
-                const int task_num = swizzle_strip(valid ? s : s1 - 1, g.TX, fr.num_tiles);
-                valid = valid && gtasks(fr.tasks)[task_num].task != 0;
+                // the unit's task words (its validity and tile come from
+                // them too: no separate load of the task record)
+                const int st = gs + tl;
+                bool valid = st < s1;
+                const int task_num = swizzle_strip(valid ? st : s1 - 1, g.TX, fr.num_tiles);
+                UnitPre up;
+                unit_centre(ca, cam, task_num, up);
+                if (color == 0) unit_extra<0>(ca, cam, up);
+                else if (color == 1) unit_extra<1>(ca, cam, up);
+                else unit_extra<2>(ca, cam, up);
+                valid = valid && up.task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
                 float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * FB;  // units 3 FB floats apart
                 if (color == 0)
-                    convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
+                    convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 else if (color == 1)
-                    convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
+                    convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 else
-                    convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB);
+                    convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 float *d = slice + li * 8;
 #pragma unroll
                 for (int qq = 0; qq < 4; qq++) {
@@ -223,7 +238,7 @@ void k_fused(FusedArgs a) {
                     *(float4 *)(d + qq * 64 + 4) = make_float4(q[4], q[5], q[6], q[7]);
                 }
                 if (fr.clt) {
-                    const int tile = task_tile(g, gtasks(fr.tasks)[task_num].txy);
+                    const int tile = task_tile(g, up.txy);
                     if (valid && tile >= 0)
                         store_clt_rows(ca.l_clt + ((size_t)tile * TP_NUM_COLORS + color) * 256, li,
                                        q0, q1, q2, q3);
