@@ -130,8 +130,15 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
     const int unit = lane >> 3, li = lane & 7;
     const int PW = g.W + 8, PH = g.H + 8;
     const int nbx = (PW + 63) / 64;
-    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
-    const int color = blockIdx.y, cam = blockIdx.z;
+    // XCD-aware order: each XCD takes a contiguous range of (block, color,
+    // camera), so neighbouring blocks, which share the tiles on their
+    // border, are converted under one L2
+    const int nblk = gridDim.x;
+    const int lin = xcd_remap(blockIdx.x + nblk * (blockIdx.y + gridDim.y * blockIdx.z),
+                              nblk * gridDim.y * gridDim.z);
+    const int blk = lin % nblk, pidx = lin / nblk;
+    const int bx = blk % nbx, by = blk / nbx;
+    const int color = pidx % gridDim.y, cam = pidx / gridDim.y;
     const int X0 = bx * 64, Y0 = by * 64;
     for (int i = threadIdx.x; i < 64 * 64; i += 256) acc[i >> 6][i & 63] = 0.f;
     __syncthreads();
