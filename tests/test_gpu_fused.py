@@ -161,6 +161,30 @@ def test_stream_matches_single_frames(stream, cfg):
     assert run.drained()
 
 
+@pytest.mark.parametrize("dct,tol", [(1, 1e-5), (2, 1e-4)])
+def test_stream_mfma_matches_fp32_stream(stream, dct, tol):
+    """The persistent stream with the split-precision matrix-core forward DCT
+    (1: FP16 hi + lo, 2: BF16 hi + lo), launched and replayed from a graph,
+    against the FP32 stream on the same frames (FP32 is within 1e-6 of the
+    FP64 oracle, so the tolerance is the split form's own bound)."""
+    fbs = stream["fbs"]
+    ref_run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=8))
+    ref_run.launch()
+    torch.cuda.synchronize()
+    refs = [fb.corrs[:6 * fb.num_tiles].cpu().numpy() for fb in fbs]
+    for fb in fbs:
+        fb.corrs.fill_(SENTINEL)
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=8, dct=dct))
+    run.capture()
+    run.replay()
+    torch.cuda.synchronize()
+    for i, fb in enumerate(fbs):
+        got = fb.corrs[:6 * fb.num_tiles].cpu().numpy()
+        assert not (got == SENTINEL).any(), (dct, i)
+        assert rel_err(got, refs[i]) <= tol, (dct, i)
+    run.release()
+
+
 def test_stream_graph_replays(stream):
     """A captured graph replayed several times, with new image data uploaded
     between replays and eager work in between: every replay matches the
