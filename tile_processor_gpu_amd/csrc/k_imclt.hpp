@@ -27,6 +27,14 @@ __device__ __forceinline__ float hw2_sel(int i) {
     return h;
 }
 
+// 1 / HWINDOW2[i] for a runtime i (the un-windowing factor of one axis)
+__device__ __forceinline__ float ihw2_sel(int i) {
+    float h = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) h = (i == t) ? 2.0f / kTab.hw[t] : h;
+    return h;
+}
+
 // Steps 2-4 on one unit; q* in: row li, out: column li of each quadrant.
 // L: the unit's staging, NQ x 72 floats; NQ quadrants are transposed at a
 // time (NQ = 1 keeps the LDS footprint of a workgroup small).
@@ -34,10 +42,17 @@ template <int NQ = 4>
 __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], float (&q2)[8],
                                                 float (&q3)[8], float *L, int li) {
     static_assert(NQ == 1 || NQ == 2 || NQ == 4, "quadrants per transposition");
+#if TP_IMCLT_FAST
+    dct4_8(q0);
+    dst4_8(q1);
+    dct4_8(q2);
+    dst4_8(q3);
+#else
     dtt8<0>(q0);
     dtt8<1>(q1);
     dtt8<0>(q2);
     dtt8<1>(q3);
+#endif
     float *q[4] = {q0, q1, q2, q3};
 #pragma unroll
     for (int q0i = 0; q0i < 4; q0i += NQ) {
@@ -54,21 +69,33 @@ __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], 
             for (int i = 0; i < 8; i++) q[q0i + j][i] = L[j * 72 + i * 8 + li];
         wave_lds_sync();
     }
+#if TP_IMCLT_FAST
+    dct4_8(q0);
+    dct4_8(q1);
+    dst4_8(q2);
+    dst4_8(q3);
+#else
     dtt8<0>(q0);
     dtt8<0>(q1);
     dtt8<1>(q2);
     dtt8<1>(q3);
+#endif
 }
 
 // Step 5: calls emit(row, col, weight, sum) for the 32 outputs of lane li.
-template <typename F>
+// weight = HWINDOW2(row) * HWINDOW2(col); with UNWIN it is
+// 1 / (HWINDOW2(row) * HWINDOW2(col)) / 16, i.e. the windowed output
+// divided by HWINDOW_SQ(row) * HWINDOW_SQ(col) (HWINDOW_SQ = HWINDOW^2 =
+// 4 HWINDOW2^2): the un-windowed pixel that debayering starts from.
+template <bool UNWIN = false, typename F>
 __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (&t1)[8],
                                              const float (&t2)[8], const float (&t3)[8], int li,
                                              F &&emit) {
     const int ca = li >= 4 ? li - 4 : 11 - li;
     const int cb = li >= 4 ? 11 - li : 12 + li;
-    const float wca = hw2_sel(ca < 8 ? ca : 15 - ca);
-    const float wcb = hw2_sel(cb < 8 ? cb : 15 - cb);
+    const int sa = ca < 8 ? ca : 15 - ca, sb = cb < 8 ? cb : 15 - cb;
+    const float wca = UNWIN ? 0.0625f * ihw2_sel(sa) : hw2_sel(sa);
+    const float wcb = UNWIN ? 0.0625f * ihw2_sel(sb) : hw2_sel(sb);
     const float cca = ca < 4 ? 1.f : -1.f, sca = ca < 12 ? 1.f : -1.f;
     const float ccb = cb < 4 ? 1.f : -1.f, scb = cb < 12 ? 1.f : -1.f;
 #pragma unroll
@@ -77,7 +104,7 @@ __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (
         const float cr = (float)fold_rcs(r), sr = (float)fold_rss(r);
         const float A = fmaf(cr, t0[rho], sr * t2[rho]);
         const float B = fmaf(cr, t1[rho], sr * t3[rho]);
-        const float wr = 0.5f * kTab.hw[r < 8 ? r : 15 - r];
+        const float wr = UNWIN ? 2.0f / kTab.hw[r < 8 ? r : 15 - r] : 0.5f * kTab.hw[r < 8 ? r : 15 - r];
         emit(r, ca, wr * wca, fmaf(cca, A, sca * B));
         emit(r, cb, wr * wcb, fmaf(ccb, A, scb * B));
     }

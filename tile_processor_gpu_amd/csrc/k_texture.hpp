@@ -7,11 +7,12 @@
 //
 // One workgroup (4 wavefronts) per texture tile:
 //   1. the 4 cameras x colors units run the inverse CLT of the low-passed
-//      CLT (8 lanes per unit), results M[cam][color][16][16] in LDS;
-//   2. thread = pixel: un-window, 3x3 debayer, shot-noise companding,
-//      re-window -> D in LDS;
-//   3. thread = pixel: 4-camera consensus (pairwise port weights, best-pair
-//      blend, per-port weights, dust removal), colour from M, alpha;
+//      CLT (8 lanes per unit); the unfold writes the un-windowed pixels
+//      U[cam][color][16][16] to LDS (the window folded into its weights);
+//   2. thread = pixel: 3x3 debayer of U, shot-noise companding,
+//      window -> D of its own pixel, in registers;
+//   3. the same thread: 4-camera consensus (pairwise port weights, best-pair
+//      blend, per-port weights, dust removal), colour from U, alpha;
 //      non-overlapping tile slices and/or the alpha-faded overlap-add into
 //      the RGBA image.
 // The list kernels replace CUDA dynamic parallelism: the host enqueues them
@@ -69,28 +70,31 @@ __device__ __forceinline__ float alpha_fade(int m, int i, int j) {
     }
 }
 
-// LDS of one texture tile.  M planes are TEX_MS floats apart: the imclt
+// LDS of one texture tile.  U planes are TEX_MS floats apart: the imclt
 // unfold writes columns {0-3, 8-11} (or {4-7, 12-15}) of one row in each of
 // 8 planes at once, and the 4-float skew spreads those over all 32 banks.
 constexpr int TEX_MS = 260;
+// Per-launch constants, computed once per workgroup: the pair factors
+// -log2(e) / (2 sigma^2 |port_j - port_i|^2) of the 6 port pairs, -log2(e) /
+// (2 sigma^2) and 1 / sqrt(min_shot).
+enum { TC_PAIR = 0, TC_KSIG = 6, TC_KSHOT = 7, TC_N = 8 };
 struct TexShared {
-    float M[12 * TEX_MS];  // [cam * colors + color][256]
-    union {
-        float scratch[2][8][4][72];  // phase 1: imclt transposes (wavefronts 0, 1)
-        float D[12 * 256];           // phase 2: debayered planes
-    } u;
+    float U[12 * TEX_MS];  // [cam * colors + color][256], un-windowed
+    float cst[TC_N];
+    float scratch[2][8][TP_TEX_NQ][72];  // phase 1: imclt transposes (wavefronts 0, 1)
 };
 
 // Per-pixel constants of thread t (pixel r = t / 16, c = t % 16), the same
 // for every tile: the window weights and the 3 x 3 debayer neighbourhood
-// (offsets, and the un-window factors of each neighbour; edge pixels use
-// their own value only).
+// (edge pixels use their own value only).
 struct TexPix {
     int r, c;
     bool edge;
     float wsq, wnd2_inv;
-    int nb[9];    // plane offsets of the neighbours (dr, dc) = (k / 3 - 1, k % 3 - 1)
-    float fw[9];  // 1 / window^2 of each neighbour
+    // plane offset of the 3 x 3 neighbourhood's corner: pixel t + 16 dr + dc
+    // is nb0 + 16 (dr + 1) + dc + 1.  Edge pixels use t itself (their
+    // neighbour reads stay inside TexShared and are not used).
+    int nb0;
 };
 
 __device__ __forceinline__ TexPix tex_pix(int t) {
@@ -101,13 +105,22 @@ __device__ __forceinline__ TexPix tex_pix(int t) {
     q.wsq = hw_sq(sym(q.r)) * hw_sq(sym(q.c));
     q.wnd2_inv = 1.0f / q.wsq;
     q.edge = q.r == 0 || q.r == 15 || q.c == 0 || q.c == 15;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const int dr = q.edge ? 0 : k / 3 - 1, dc = q.edge ? 0 : k % 3 - 1;
-        q.nb[k] = t + 16 * dr + dc;
-        q.fw[k] = (1.0f / hw_sq(sym(q.c + dc))) * (1.0f / hw_sq(sym(q.r + dr)));
-    }
+    q.nb0 = q.edge ? t : t - 17;
     return q;
+}
+
+// TexShared::cst (thread 0; the caller synchronises)
+__device__ __forceinline__ void tex_constants(const TexArgs &a, float *cst) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    const float ksigma = 1.0f / (2.0f * a.diff_sigma * a.diff_sigma);
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++, k++) {
+            const float dx = a.ports[2 * j] - a.ports[2 * i], dy = a.ports[2 * j + 1] - a.ports[2 * i + 1];
+            cst[TC_PAIR + k] = -kLog2e * (ksigma / (dx * dx + dy * dy));
+        }
+    cst[TC_KSIG] = -kLog2e * ksigma;
+    cst[TC_KSHOT] = 1.0f / sqrtf(a.min_shot);
 }
 
 // One texture list entry e (of n) by the whole workgroup; returns early
@@ -115,8 +128,7 @@ __device__ __forceinline__ TexPix tex_pix(int t) {
 template <int COLORS>
 __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, TexShared &sh,
                                              const TexPix &px) {
-    float *M = sh.M;
-    auto &u = sh.u;
+    float *U = sh.U;
     const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
     const int code = list[e];
     if (!(code & (1 << TP_LIST_TEXTURE_BIT))) return;
@@ -126,7 +138,7 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = wave * 8 + (lane >> 3), li = lane & 7;
 
-    // phase 1: M = imclt(CLT * lpf)
+    // phase 1: U = imclt(CLT * lpf), un-windowed
     if (slot < 4 * colors) {
         const int cam = slot / colors, col = slot % colors;
         const int lw = colors > 1 ? col : 3;
@@ -137,25 +149,31 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
             const float l = c_lpf.v[lw][li * 8 + k];
             q0[k] *= l; q1[k] *= l; q2[k] *= l; q3[k] *= l;
         }
-        imclt_transform(q0, q1, q2, q3, &u.scratch[wave][lane >> 3][0][0], li);
-        float *m = M + slot * TEX_MS;
-        imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
+        imclt_transform<TP_TEX_NQ>(q0, q1, q2, q3, &sh.scratch[wave][lane >> 3][0][0], li);
+        float *m = U + slot * TEX_MS;
+        imclt_unfold<true>(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
     }
     __syncthreads();
 
     const int t = threadIdx.x, r = px.r, c = px.c;
     const float wsq = px.wsq;
-    // phase 2: debayer_shot (colors > 1): un-window, 3 x 3 debayer (edge
-    // pixels unchanged), shot-noise companding, re-window; else D = M
+    // phase 2: debayer_shot (colors > 1): 3 x 3 debayer of the un-windowed
+    // pixels (edge pixels unchanged), shot-noise companding, window; else
+    // D = windowed U.  Thread t keeps D and U of its own pixel.
+    float D[4][3], Uv[4][3];
+#pragma unroll
+    for (int cam = 0; cam < 4; cam++)
+#pragma unroll
+        for (int col = colors; col < 3; col++) D[cam][col] = Uv[cam][col] = 0.f;
     if (colors > 1) {
-        const float k = 1.0f / sqrtf(a.min_shot);
+        const float k = sh.cst[TC_KSHOT];
 #pragma unroll
         for (int s = 0; s < 4 * colors; s++) {
-            const float *x = M + s * TEX_MS;
+            const float *x = U + s * TEX_MS;
             const int col = s % colors;
             float v[9];
 #pragma unroll
-            for (int j = 0; j < 9; j++) v[j] = x[px.nb[j]] * px.fw[j];
+            for (int j = 0; j < 9; j++) v[j] = j == 4 ? x[t] : x[px.nb0 + 16 * (j / 3) + j % 3];
             const float cross = (v[1] + v[7]) + (v[3] + v[5]);
             float d;
             if (col < 2) {
@@ -165,25 +183,20 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
                 d = fmaf(0.125f, cross, 0.5f * v[4]);
             }
             d = px.edge ? v[4] : d;
-            if (a.scale_shot > 0.0f) d = a.scale_shot * (d > a.min_shot ? sqrtf(d) : k * d);
-            u.D[s * 256 + t] = d * wsq;
+            if (a.scale_shot > 0.0f) d = a.scale_shot * (d > a.min_shot ? __builtin_amdgcn_sqrtf(d) : k * d);
+            D[s / colors][col] = d * wsq;
+            Uv[s / colors][col] = v[4];
         }
     } else {
-        for (int s = 0; s < 4; s++) u.D[s * 256 + t] = M[s * TEX_MS + t];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            Uv[s][0] = U[s * TEX_MS + t];
+            D[s][0] = Uv[s][0] * wsq;
+        }
     }
-    __syncthreads();
 
     // phase 3: consensus per pixel
-    float D[4][3], Mv[4][3];
-#pragma unroll
-    for (int cam = 0; cam < 4; cam++)
-#pragma unroll
-        for (int col = 0; col < 3; col++) {
-            D[cam][col] = col < colors ? u.D[(cam * colors + col) * 256 + t] : 0.f;
-            Mv[cam][col] = col < colors ? M[(cam * colors + col) * TEX_MS + t] : 0.f;
-        }
-    const float ksigma = 1.0f / (2.0f * a.diff_sigma * a.diff_sigma);
-    const float wnd2_inv = px.wnd2_inv;
+    const float *cst = sh.cst;
     float out[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) out[i] = 0.f;
@@ -207,19 +220,24 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
         }
         crms[colors] = sqrtf(sw);
     }
+    // The colour distances below are sum_col w[col] ((x - y) / window^2)^2;
+    // D is divided by window^2 once.
+#pragma unroll
+    for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int cam = 0; cam < 4; cam++) D[cam][col] *= px.wnd2_inv;
     float p[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0, k = 0; i < 4; i++)
 #pragma unroll
-        for (int j = i + 1; j < 4; j++) {
-            const float dx = a.ports[2 * j] - a.ports[2 * i], dy = a.ports[2 * j + 1] - a.ports[2 * i + 1];
-            const float dist2r = ksigma / (dx * dx + dy * dy);
+        for (int j = i + 1; j < 4; j++, k++) {
             float d = 0.f;
+#pragma unroll
             for (int col = 0; col < colors; col++) {
-                const float dc = (D[i][col] - D[j][col]) * wnd2_inv;
+                const float dc = D[i][col] - D[j][col];
                 d = fmaf(a.w[col] * dc, dc, d);
             }
-            d = __expf(-dist2r * d) + (float)TP_FAT_ZERO_WEIGHT;
+            d = __builtin_amdgcn_exp2f(cst[TC_PAIR + k] * d) + (float)TP_FAT_ZERO_WEIGHT;
             p[i] += d;
             p[j] += d;
         }
@@ -235,7 +253,7 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
         if (cam > (b1 == 0 ? 1 : 0) && cam != b1 && p[cam] > best) { b2 = cam; best = p[cam]; }
     const float pb1 = b1 == 0 ? p[0] : b1 == 1 ? p[1] : b1 == 2 ? p[2] : p[3];
     const float pb2 = b2 == 0 ? p[0] : b2 == 1 ? p[1] : b2 == 2 ? p[2] : p[3];
-    const float w1 = pb1 / (pb1 + pb2), w2 = 1.f - w1;
+    const float w1 = pb1 * __builtin_amdgcn_rcpf(pb1 + pb2), w2 = 1.f - w1;
     float blend[3];
 #pragma unroll
     for (int col = 0; col < 3; col++) {
@@ -250,11 +268,12 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
 #pragma unroll
     for (int cam = 0; cam < 4; cam++) {
         float d2 = 0.f;
+#pragma unroll
         for (int col = 0; col < colors; col++) {
-            const float dc = (D[cam][col] - blend[col]) * wnd2_inv;
+            const float dc = D[cam][col] - blend[col];
             d2 = fmaf(a.w[col] * dc, dc, d2);
         }
-        p[cam] = __expf(-ksigma * d2) + (float)TP_FAT_ZERO_WEIGHT;
+        p[cam] = __builtin_amdgcn_exp2f(cst[TC_KSIG] * d2) + (float)TP_FAT_ZERO_WEIGHT;
     }
     if (a.dust_remove) {
         int worst = 0;
@@ -263,22 +282,23 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
         for (int cam = 1; cam < 4; cam++)
             if (p[cam] < wv) { worst = cam; wv = p[cam]; }
         const float avg = ((p[0] + p[1]) + (p[2] + p[3]) - wv) * (1.0f / 3.0f);
-        const float scale = 1.f + wv * (avg - wv) / (avg * avg * 3.f);
+        const float ia = __builtin_amdgcn_rcpf(avg);
+        const float scale = 1.f + wv * (avg - wv) * (ia * ia) * (1.0f / 3.0f);
+        const float fworst = wv * ia;
 #pragma unroll
-        for (int cam = 0; cam < 4; cam++) p[cam] *= (cam == worst) ? wv / avg : scale;
+        for (int cam = 0; cam < 4; cam++) p[cam] *= (cam == worst) ? fworst : scale;
     }
     const float psum = (p[0] + p[1]) + (p[2] + p[3]);
-    const float kn = 1.f / psum;
+    const float kn = wsq * __builtin_amdgcn_rcpf(psum);  // the window of U folded in
     for (int col = 0; col < colors; col++) {
         float acc = 0.f;
 #pragma unroll
-        for (int cam = 0; cam < 4; cam++) acc = fmaf(kn * p[cam], Mv[cam][col], acc);
-        rgba[col] = acc;
+        for (int cam = 0; cam < 4; cam++) acc = fmaf(p[cam], Uv[cam][col], acc);
+        rgba[col] = kn * acc;
     }
     out[colors] = wsq * psum * 0.25f;
 #pragma unroll
     for (int cam = 0; cam < 4; cam++) pw[cam] = p[cam];
-
     const int nsl_all = 2 * colors + 6;
     if (a.tiles && a.tex_stride) {
         const int nsl = a.keep_weights ? nsl_all : colors + 1;
@@ -297,10 +317,13 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
 // A fixed grid of workgroups loops over the list (its length may only be
 // known on the device), one entry at a time.
 template <int COLORS>
-__global__ __launch_bounds__(256) void k_textures_accumulate(TexArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
+void k_textures_accumulate(TexArgs a) {
     __shared__ __align__(16) TexShared sh;
     const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
     const TexPix px = tex_pix(threadIdx.x);
+    if (threadIdx.x == 0) tex_constants(a, sh.cst);
+    __syncthreads();
     for (int e = blockIdx.x; e < n; e += gridDim.x) {
         texture_tile<COLORS>(a, e, n, sh, px);
         __syncthreads();  // the LDS of this entry is rewritten by the next one

@@ -305,10 +305,9 @@ namespace {
 
 int launch_textures(const tp::TexArgs &a, int grid, hipStream_t st) {
     if (grid <= 0) return TP_OK;
-    // workgroups loop over the list: at most 5 resident per CU (31 KB of
-    // LDS each), a few rounds of them
+    // workgroups loop over the list, TP_TEX_WG_PER_CU per CU
     const int cus = num_cus();
-    grid = std::min(grid, (cus > 0 ? cus : 256) * 5 * 4);
+    grid = std::min(grid, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
     if (a.colors == 3)
         hipLaunchKernelGGL(tp::k_textures_accumulate<3>, dim3(grid), dim3(256), 0, st, a);
     else

@@ -74,6 +74,18 @@
 #ifndef TP_IMCLT_WAVES
 #define TP_IMCLT_WAVES 5      // whole-frame inverse CLT: minimum waves per SIMD
 #endif
+#ifndef TP_IMCLT_FAST
+#define TP_IMCLT_FAST 1       // inverse CLT: DCT-IV / DST-IV through a 4-point complex DFT (0: matrix form)
+#endif
+#ifndef TP_TEX_WG_PER_CU
+#define TP_TEX_WG_PER_CU 5    // textures: workgroups per CU in the (list-looping) grid
+#endif
+#ifndef TP_TEX_NQ
+#define TP_TEX_NQ 1           // textures: quadrants per LDS transposition of the inverse CLT (1, 2, 4)
+#endif
+#ifndef TP_TEX_WAVES
+#define TP_TEX_WAVES 4        // textures: minimum waves per SIMD
+#endif
 #ifndef TP_KERN_SHARE
 #define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
 #endif
