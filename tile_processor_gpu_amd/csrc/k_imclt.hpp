@@ -94,10 +94,13 @@ __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (
     const int ca = li >= 4 ? li - 4 : 11 - li;
     const int cb = li >= 4 ? 11 - li : 12 + li;
     const int sa = ca < 8 ? ca : 15 - ca, sb = cb < 8 ? cb : 15 - cb;
-    const float wca = UNWIN ? 0.0625f * ihw2_sel(sa) : hw2_sel(sa);
-    const float wcb = UNWIN ? 0.0625f * ihw2_sel(sb) : hw2_sel(sb);
+    // column c's sum is cc A + sc B (signs cc, sc = +-1) = cc (A + cc sc B):
+    // cc goes into the column weight, exactly
     const float cca = ca < 4 ? 1.f : -1.f, sca = ca < 12 ? 1.f : -1.f;
     const float ccb = cb < 4 ? 1.f : -1.f, scb = cb < 12 ? 1.f : -1.f;
+    const float wca = cca * (UNWIN ? 0.0625f * ihw2_sel(sa) : hw2_sel(sa));
+    const float wcb = ccb * (UNWIN ? 0.0625f * ihw2_sel(sb) : hw2_sel(sb));
+    const float rha = cca * sca, rhb = ccb * scb;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int rho = fold_rind(r);
@@ -105,8 +108,8 @@ __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (
         const float A = fmaf(cr, t0[rho], sr * t2[rho]);
         const float B = fmaf(cr, t1[rho], sr * t3[rho]);
         const float wr = UNWIN ? 2.0f / kTab.hw[r < 8 ? r : 15 - r] : 0.5f * kTab.hw[r < 8 ? r : 15 - r];
-        emit(r, ca, wr * wca, fmaf(cca, A, sca * B));
-        emit(r, cb, wr * wcb, fmaf(ccb, A, scb * B));
+        emit(r, ca, wr * wca, fmaf(rha, B, A));
+        emit(r, cb, wr * wcb, fmaf(rhb, B, A));
     }
 }
 
