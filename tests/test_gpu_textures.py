@@ -84,14 +84,14 @@ def test_texture_tiles_match_oracle(tex, orc, keep):
         assert fr >= 0.9999, (s, fr, worst)
 
 
-def _gpu_generate(tex, colors=3):
+def _gpu_generate(tex, colors=3, pad=8):
     g = tex["g"]
     t = np.ascontiguousarray(tex["tasks"])
     d_tasks = torch.from_numpy(t.view(np.uint8).copy()).cuda()
     ind = torch.full((g.tilesx * g.tilesya,), -1, dtype=torch.int32, device="cuda:0")
     num = torch.full((8,), -1, dtype=torch.int32, device="cuda:0")
     woi = torch.full((4,), -1, dtype=torch.int32, device="cuda:0")
-    stride = (g.tilesx + 1) * 8 + 8
+    stride = (g.tilesx + 1) * 8 + pad
     rbga = torch.full(((colors + 1) * (g.tilesy + 1) * 8, stride), 7.0, device="cuda:0")
     kw = dict(tp.TEXTURE_DEFAULTS, colors=colors)
     tp.generate_RBGA(g, d_tasks, len(t), ind, num, woi, g.tilesx, g.tilesy, tex["p_clt"],
@@ -103,9 +103,10 @@ def _gpu_generate(tex, colors=3):
     return tasks_out, ind.cpu().numpy(), num.cpu().numpy(), woi.cpu().numpy(), rbga.cpu().numpy(), stride
 
 
-def test_generate_rbga_matches_oracle(tex, orc):
+@pytest.mark.parametrize("pad", [8, 3])  # 3: a row stride that rules out 16-byte stores
+def test_generate_rbga_matches_oracle(tex, orc, pad):
     g = tex["g"]
-    tasks_out, ind, num, woi, rbga, stride = _gpu_generate(tex)
+    tasks_out, ind, num, woi, rbga, stride = _gpu_generate(tex, pad=pad)
     t_ref, woi_ref, num_ref, lists_ref, rbga_ref = orc.generate_rbga(g, tex["tasks"], tex["clt64"],
                                                                      orc.TexParams(), stride)
     assert list(woi) == list(woi_ref)

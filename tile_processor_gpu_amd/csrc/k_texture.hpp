@@ -466,13 +466,27 @@ __global__ void k_tex_finish_woi(int *woi) {
 }
 
 // zero the RGBA image rows of the woi: (woi_w + 1) * 8 columns,
-// (woi_h + 1) * 8 rows per slice; grid covers the largest possible woi
-__global__ void k_tex_clear_rbga(float *rbga, size_t stride, const int *woi, int slices) {
+// (woi_h + 1) * 8 rows per slice (slice-major rows).  One workgroup per
+// TP_CLEAR_ROWS rows, grid for the largest possible woi; VEC: 16-byte
+// stores (the host checks that the base and the stride allow them).
+#define TP_CLEAR_ROWS 8
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_tex_clear_rbga(float *rbga, size_t stride, const int *woi,
+                                                        int slices) {
     const int wpx = (woi[2] + 1) * 8, hpx = (woi[3] + 1) * 8;
-    const int row = blockIdx.y;  // slice-major rows
-    if (row >= hpx * slices) return;
-    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < wpx; x += gridDim.x * blockDim.x)
-        rbga[(size_t)row * stride + x] = 0.f;
+    const int row0 = blockIdx.x * TP_CLEAR_ROWS;
+    const int nrow = min(TP_CLEAR_ROWS, hpx * slices - row0);
+    if (nrow <= 0) return;
+    const int q = VEC ? wpx / 4 : wpx;  // wpx is a multiple of 8
+    for (int r = 0; r < nrow; r++) {
+        float *p = rbga + (size_t)(row0 + r) * stride;
+        for (int x = threadIdx.x; x < q; x += 256) {
+            if (VEC)
+                ((float4 *)p)[x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            else
+                p[x] = 0.f;
+        }
+    }
 }
 
 }  // namespace tp

@@ -471,8 +471,13 @@ int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
     tp::Geo g = tp::make_geo(*geom);
     const int slices = colors + 1;
     const int max_rows = (g.TY + 1) * 8 * slices;
-    hipLaunchKernelGGL(tp::k_tex_clear_rbga, dim3(((g.TX + 1) * 8 + 255) / 256, max_rows), dim3(256),
-                       0, st, gpu_texture_tiles, texture_rbga_stride, woi, slices);
+    const dim3 cgrid((max_rows + TP_CLEAR_ROWS - 1) / TP_CLEAR_ROWS);
+    if (texture_rbga_stride % 4 == 0 && (uintptr_t)gpu_texture_tiles % 16 == 0)
+        hipLaunchKernelGGL(tp::k_tex_clear_rbga<true>, cgrid, dim3(256), 0, st, gpu_texture_tiles,
+                           texture_rbga_stride, woi, slices);
+    else
+        hipLaunchKernelGGL(tp::k_tex_clear_rbga<false>, cgrid, dim3(256), 0, st, gpu_texture_tiles,
+                           texture_rbga_stride, woi, slices);
     if ((rc = check_launch("clear_texture_rbga"))) return rc;
     tp::TexArgs a = tex_args(geom, woi, gpu_clt, gpu_port_offsets, colors, min_shot, scale_shot,
                              diff_sigma, diff_threshold, min_agree, weight0, weight1, weight2,
