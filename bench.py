@@ -286,6 +286,33 @@ def measure_stream(nframes, reps, seed=4):
                             "max_rel_dev_vs_fp32_stream": dev16}}
 
 
+def make_texture(cfg, fb):
+    """generate_RBGA of the whole frame from fb's CLT into an RGBA image
+    (the task words, which it rewrites, restored first)."""
+    import torch
+    from tile_processor_gpu_amd import tp
+    g = cfg["geom"]
+    dev = torch.device("cuda:0")
+    t = np.ascontiguousarray(cfg["tasks"])
+    tasks0 = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
+    tasks = tasks0.clone()
+    ind = torch.zeros(g.tilesx * g.tilesya, dtype=torch.int32, device=dev)
+    num = torch.zeros(8, dtype=torch.int32, device=dev)
+    woi = torch.zeros(4, dtype=torch.int32, device=dev)
+    stride = (g.tilesx + 1) * 8
+    rbga = torch.zeros((4 * (g.tilesy + 1) * 8, stride), device=dev)
+    ports = torch.tensor([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]], device=dev)
+    kw = tp.TEXTURE_DEFAULTS
+
+    def texture():
+        tasks.copy_(tasks0)
+        tp.generate_RBGA(g, tasks, len(t), ind, num, woi, g.tilesx, g.tilesy, fb.p_clt, ports,
+                         3, 0, kw["min_shot"], kw["scale_shot"], kw["diff_sigma"], kw["diff_threshold"],
+                         kw["min_agree"], kw["weight0"], kw["weight1"], kw["weight2"],
+                         kw["dust_remove"], 0, stride, rbga)
+    return texture
+
+
 def measure_stages(cfg, fb, idx, reps):
     """The rest of the full-frame pipeline, timed separately (not part of
     `value`): six-pair combination to 15 x 15 tiles, texture output
@@ -332,23 +359,7 @@ def measure_stages(cfg, fb, idx, reps):
     out["imclt_48_launches_ms"] = round(median_ms(imclt_passes, max(3, reps // 4), warmup=1), 4)
     del planes
     # texture output
-    t = np.ascontiguousarray(cfg["tasks"])
-    tasks0 = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
-    tasks = tasks0.clone()
-    ind = torch.zeros(g.tilesx * g.tilesya, dtype=torch.int32, device=dev)
-    num = torch.zeros(8, dtype=torch.int32, device=dev)
-    woi = torch.zeros(4, dtype=torch.int32, device=dev)
-    stride = (g.tilesx + 1) * 8
-    rbga = torch.zeros((4 * (g.tilesy + 1) * 8, stride), device=dev)
-    ports = torch.tensor([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]], device=dev)
-    kw = tp.TEXTURE_DEFAULTS
-
-    def texture():
-        tasks.copy_(tasks0)
-        tp.generate_RBGA(g, tasks, len(t), ind, num, woi, g.tilesx, g.tilesy, fb.p_clt, ports,
-                         3, 0, kw["min_shot"], kw["scale_shot"], kw["diff_sigma"], kw["diff_threshold"],
-                         kw["min_agree"], kw["weight0"], kw["weight1"], kw["weight2"],
-                         kw["dust_remove"], 0, stride, rbga)
+    texture = make_texture(cfg, fb)
     out["texture_rgba_ms"] = round(median_ms(texture, max(3, reps // 2)), 4)
 
     # the whole per-frame pipeline (forward CLT, 6-pair correlation, 15 x 15
@@ -402,9 +413,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the plain staged path (no HBM ceilings, fused or stream measurements)")
     ap.add_argument("--stream-frames", type=int, default=64)
-    ap.add_argument("--mode", choices=["frame", "copy", "fused"], default="frame",
+    ap.add_argument("--mode", choices=["frame", "copy", "fused", "texture"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
-                         "FETCH_SIZE/WRITE_SIZE against known bytes)")
+                         "FETCH_SIZE/WRITE_SIZE against known bytes); fused: the fused kernel "
+                         "alone; texture: generate_RBGA, then the whole-frame inverse CLT, each "
+                         "--steps times on the full frame (for profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -435,6 +448,23 @@ def main():
         fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
         ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO), max(1, args.steps), args.warmup)
         print(json.dumps({"mode": "fused", "ms": round(ms, 4), "steps": args.steps}), flush=True)
+        return
+
+    if args.mode == "texture":
+        cfg = synth.config_full(seed=args.seed)
+        g = cfg["geom"]
+        idx = corr_indices(g, cfg["tasks"])
+        fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
+        fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+        fb.convert()
+        ms_tex = median_ms(make_texture(cfg, fb), max(1, args.steps), args.warmup)
+        dstride = g.img_width + 8
+        planes = [torch.zeros(3 * (g.img_height + 8) * dstride, device="cuda:0") for _ in range(4)]
+        p_planes = tp.ptr_array(planes)
+        ms_imclt = median_ms(lambda: tp.imclt_rbg_all(g, fb.p_clt, p_planes, dstride),
+                             max(1, args.steps), args.warmup)
+        print(json.dumps({"mode": "texture", "texture_rgba_ms": round(ms_tex, 4),
+                          "imclt_frame_ms": round(ms_imclt, 4), "steps": args.steps}), flush=True)
         return
 
     t0 = time.time()
