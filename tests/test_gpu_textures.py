@@ -50,18 +50,19 @@ def frac_close(got, ref, tol=TOL):
     return 1.0 - bad.mean(), float(np.abs(got - ref).max() / den)
 
 
+@pytest.mark.parametrize("colors", [3, 1])
 @pytest.mark.parametrize("keep", [0, 1])
-def test_texture_tiles_match_oracle(tex, orc, keep):
+def test_texture_tiles_match_oracle(tex, orc, keep, colors):
     g = tex["g"]
     idx = texture_indices(g, tex["tasks"])
-    prm = orc.TexParams(keep_weights=keep)
+    prm = orc.TexParams(keep_weights=keep, colors=colors)
     nsl = prm.slices()
     stride = nsl * 256 + 64
     ref = np.zeros((len(idx), stride))
     orc.textures_accumulate(g, None, tex["clt64"], idx, prm, tiles=ref, tex_stride=stride)
     d_idx = torch.from_numpy(idx).cuda()
     out = torch.full((len(idx), stride), -9.0, device="cuda:0")
-    kw = dict(tp.TEXTURE_DEFAULTS, keep_weights=keep)
+    kw = dict(tp.TEXTURE_DEFAULTS, keep_weights=keep, colors=colors)
     tp.textures_accumulate(g, None, tex["p_clt"], len(idx), d_idx, tex["ports"],
                            kw["colors"], kw["is_lwir"], kw["min_shot"], kw["scale_shot"],
                            kw["diff_sigma"], kw["diff_threshold"], kw["min_agree"], kw["weight0"],
@@ -69,15 +70,15 @@ def test_texture_tiles_match_oracle(tex, orc, keep):
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     assert (got[:, nsl * 256:] == -9.0).all()
-    # port weights (slices 4..7) enter the output only through alpha =
-    # w^2 sum(weights) / 4; at tile corners they are ill-conditioned by
-    # construction (colour differences are scaled by 1 / w^4 before the
-    # exponential), so they are compared as w^2 * weight
+    # port weights (slices colors + 1 .. colors + 4) enter the output only
+    # through alpha = w^2 sum(weights) / 4; at tile corners they are
+    # ill-conditioned by construction (colour differences are scaled by
+    # 1 / w^4 before the exponential), so they are compared as w^2 * weight
     w = np.sin(np.pi * (np.arange(16) + 0.5) / 16) ** 2
     w2 = np.outer(w, w).ravel()
     for s in range(nsl):
         a, b = got[:, s * 256:(s + 1) * 256], ref[:, s * 256:(s + 1) * 256]
-        if 4 <= s < 8:
+        if colors + 1 <= s < colors + 5:
             a, b = a * w2, b * w2
         fr, worst = frac_close(a, b)
         print(f"slice {s}: close {fr:.6f}, worst {worst:.2e}")

@@ -70,18 +70,20 @@ __device__ __forceinline__ float alpha_fade(int m, int i, int j) {
     }
 }
 
-// LDS of one texture tile.  U planes are TEX_MS floats apart: the imclt
-// unfold writes columns {0-3, 8-11} (or {4-7, 12-15}) of one row in each of
-// 8 planes at once, and the 4-float skew spreads those over all 32 banks.
+// LDS of the workgroup's TP_TEX_TPW texture tiles.  U planes are TEX_MS
+// floats apart: the imclt unfold writes columns {0-3, 8-11} (or {4-7,
+// 12-15}) of one row in each of 8 planes at once, and the 4-float skew
+// spreads those over all 32 banks.
 constexpr int TEX_MS = 260;
+constexpr int TEX_UNIT_WAVES = (TP_TEX_TPW * 12 + 7) / 8;  // wavefronts of phase 1 (3 colors)
 // Per-launch constants, computed once per workgroup: the pair factors
 // -log2(e) / (2 sigma^2 |port_j - port_i|^2) of the 6 port pairs, -log2(e) /
 // (2 sigma^2) and 1 / sqrt(min_shot).
 enum { TC_PAIR = 0, TC_KSIG = 6, TC_KSHOT = 7, TC_N = 8 };
 struct TexShared {
-    float U[12 * TEX_MS];  // [cam * colors + color][256], un-windowed
+    float U[TP_TEX_TPW * 12 * TEX_MS];  // [tile][cam * colors + color][256], un-windowed
     float cst[TC_N];
-    float scratch[2][8][TP_TEX_NQ][72];  // phase 1: imclt transposes (wavefronts 0, 1)
+    float scratch[TEX_UNIT_WAVES][8][TP_TEX_NQ][72];  // phase 1: imclt transposes
 };
 
 // Per-pixel constants of thread t (pixel r = t / 16, c = t % 16), the same
@@ -123,38 +125,12 @@ __device__ __forceinline__ void tex_constants(const TexArgs &a, float *cst) {
     cst[TC_KSHOT] = 1.0f / sqrtf(a.min_shot);
 }
 
-// One texture list entry e (of n) by the whole workgroup; returns early
-// (uniformly) for entries without a texture bit or off the grid.
+// Phases 2 and 3 of list entry e (code, tile) for thread t = pixel; U: the
+// entry's 4 x colors un-windowed planes in LDS.
 template <int COLORS>
-__device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, TexShared &sh,
-                                             const TexPix &px) {
-    float *U = sh.U;
-    const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
-    const int code = list[e];
-    if (!(code & (1 << TP_LIST_TEXTURE_BIT))) return;
-    const int tile = code >> TP_CORR_NTILE_SHIFT;
-    if (tile < 0 || tile >= a.g.TX * a.g.TY) return;
+__device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code, int tile,
+                                              const float *U, const float *cst, const TexPix &px) {
     constexpr int colors = COLORS;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slot = wave * 8 + (lane >> 3), li = lane & 7;
-
-    // phase 1: U = imclt(CLT * lpf), un-windowed
-    if (slot < 4 * colors) {
-        const int cam = slot / colors, col = slot % colors;
-        const int lw = colors > 1 ? col : 3;
-        float q0[8], q1[8], q2[8], q3[8];
-        load_rows(a.clt[cam] + ((size_t)tile * TP_NUM_COLORS + col) * 256, li, q0, q1, q2, q3);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float l = c_lpf.v[lw][li * 8 + k];
-            q0[k] *= l; q1[k] *= l; q2[k] *= l; q3[k] *= l;
-        }
-        imclt_transform<TP_TEX_NQ>(q0, q1, q2, q3, &sh.scratch[wave][lane >> 3][0][0], li);
-        float *m = U + slot * TEX_MS;
-        imclt_unfold<true>(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
-    }
-    __syncthreads();
-
     const int t = threadIdx.x, r = px.r, c = px.c;
     const float wsq = px.wsq;
     // phase 2: debayer_shot (colors > 1): 3 x 3 debayer of the un-windowed
@@ -166,7 +142,7 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
 #pragma unroll
         for (int col = colors; col < 3; col++) D[cam][col] = Uv[cam][col] = 0.f;
     if (colors > 1) {
-        const float k = sh.cst[TC_KSHOT];
+        const float k = cst[TC_KSHOT];
 #pragma unroll
         for (int s = 0; s < 4 * colors; s++) {
             const float *x = U + s * TEX_MS;
@@ -196,7 +172,6 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
     }
 
     // phase 3: consensus per pixel
-    const float *cst = sh.cst;
     float out[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) out[i] = 0.f;
@@ -314,8 +289,61 @@ __device__ __forceinline__ void texture_tile(const TexArgs &a, int e, int n, Tex
     for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
 }
 
+// List entries e0 .. e0 + TP_TEX_TPW - 1 (of n) by the whole workgroup:
+// phase 1 runs the units of all its tiles at once (8 lanes each), then the
+// threads take the tiles' pixels one tile after the other.  Entries without
+// a texture bit, off the grid or past n are skipped (uniformly).
+template <int COLORS>
+__device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared &sh,
+                                              const TexPix &px) {
+    constexpr int colors = COLORS, UPT = 4 * COLORS;  // units per tile
+    const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
+    int code[TP_TEX_TPW], tile[TP_TEX_TPW];
+    bool ok[TP_TEX_TPW], any = false;
+#pragma unroll
+    for (int k = 0; k < TP_TEX_TPW; k++) {
+        code[k] = e0 + k < n ? list[e0 + k] : 0;
+        tile[k] = code[k] >> TP_CORR_NTILE_SHIFT;
+        ok[k] = (code[k] & (1 << TP_LIST_TEXTURE_BIT)) && tile[k] >= 0 && tile[k] < a.g.TX * a.g.TY;
+        any = any || ok[k];
+    }
+    if (!any) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = wave * 8 + (lane >> 3), li = lane & 7;
+
+    // phase 1: U = imclt(CLT * lpf), un-windowed
+    if (slot < TP_TEX_TPW * UPT) {
+        const int k = slot / UPT, u = slot - k * UPT;
+        const int cam = u / colors, col = u % colors;
+        const int lw = colors > 1 ? col : 3;
+        bool okk = ok[0];
+        int tk = tile[0];
+#pragma unroll
+        for (int j = 1; j < TP_TEX_TPW; j++) {
+            okk = k == j ? ok[j] : okk;
+            tk = k == j ? tile[j] : tk;
+        }
+        if (okk) {
+            float q0[8], q1[8], q2[8], q3[8];
+            load_rows(a.clt[cam] + ((size_t)tk * TP_NUM_COLORS + col) * 256, li, q0, q1, q2, q3);
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const float l = c_lpf.v[lw][li * 8 + kk];
+                q0[kk] *= l; q1[kk] *= l; q2[kk] *= l; q3[kk] *= l;
+            }
+            imclt_transform<TP_TEX_NQ>(q0, q1, q2, q3, &sh.scratch[wave][lane >> 3][0][0], li);
+            float *m = sh.U + slot * TEX_MS;
+            imclt_unfold<true>(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TP_TEX_TPW; k++)
+        if (ok[k]) texture_pixel<COLORS>(a, e0 + k, code[k], tile[k], sh.U + k * UPT * TEX_MS, sh.cst, px);
+}
+
 // A fixed grid of workgroups loops over the list (its length may only be
-// known on the device), one entry at a time.
+// known on the device), TP_TEX_TPW entries at a time.
 template <int COLORS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
 void k_textures_accumulate(TexArgs a) {
@@ -324,9 +352,9 @@ void k_textures_accumulate(TexArgs a) {
     const TexPix px = tex_pix(threadIdx.x);
     if (threadIdx.x == 0) tex_constants(a, sh.cst);
     __syncthreads();
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
-        texture_tile<COLORS>(a, e, n, sh, px);
-        __syncthreads();  // the LDS of this entry is rewritten by the next one
+    for (int e = blockIdx.x * TP_TEX_TPW; e < n; e += gridDim.x * TP_TEX_TPW) {
+        texture_tiles<COLORS>(a, e, n, sh, px);
+        __syncthreads();  // the LDS of these entries is rewritten by the next ones
     }
 }
 

@@ -83,6 +83,9 @@
 #ifndef TP_TEX_NQ
 #define TP_TEX_NQ 1           // textures: quadrants per LDS transposition of the inverse CLT (1, 2, 4)
 #endif
+#ifndef TP_TEX_TPW
+#define TP_TEX_TPW 2          // textures: list entries (tiles) per workgroup at a time (1, 2)
+#endif
 #ifndef TP_TEX_WAVES
 #define TP_TEX_WAVES 4        // textures: minimum waves per SIMD
 #endif
