@@ -529,7 +529,7 @@ int tp_corr_combine(const tp_geometry *geom, size_t num_corr_tiles, const int *g
     a.ntiles = (geom->img_width / 8) * (geom->img_height / 8);
     a.out = gpu_combined;
     a.out_stride = combined_stride;
-    hipLaunchKernelGGL(tp::k_corr_combine, dim3((a.num_corr + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL(tp::k_corr_combine, dim3((a.num_corr + TP_COMBINE_EPW - 1) / TP_COMBINE_EPW), dim3(256), 0,
                        as_stream(stream), a);
     return check_launch("corr_combine");
 }

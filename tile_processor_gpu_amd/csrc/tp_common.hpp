@@ -83,6 +83,12 @@
 #ifndef TP_TEX_NQ
 #define TP_TEX_NQ 1           // textures: quadrants per LDS transposition of the inverse CLT (1, 2, 4)
 #endif
+#ifndef TP_COMBINE_EPW
+#define TP_COMBINE_EPW 64     // pair combination: list entries per workgroup
+#endif
+#ifndef TP_COMBINE_PIPE
+#define TP_COMBINE_PIPE 1     // pair combination: next group's rows requested during the current one
+#endif
 #ifndef TP_TEX_TPW
 #define TP_TEX_TPW 2          // textures: list entries (tiles) per workgroup at a time (1, 2)
 #endif
