@@ -15,6 +15,8 @@
 // with the sums kept in LDS, and writes the block once.
 #pragma once
 
+#include <type_traits>
+
 #include "tp_common.hpp"
 
 namespace tp {
@@ -82,12 +84,14 @@ __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], 
 #endif
 }
 
-// Step 5: calls emit(row, col, weight, sum) for the 32 outputs of lane li.
+// Step 5: calls emit(row, col, weight, sum) for the 32 outputs of lane li
+// (with SLOT: emit(row, col, slot, weight, sum), slot a std::integral_constant
+// 0 for the lane's first column, 1 for its second).
 // weight = HWINDOW2(row) * HWINDOW2(col); with UNWIN it is
 // 1 / (HWINDOW2(row) * HWINDOW2(col)) / 16, i.e. the windowed output
 // divided by HWINDOW_SQ(row) * HWINDOW_SQ(col) (HWINDOW_SQ = HWINDOW^2 =
 // 4 HWINDOW2^2): the un-windowed pixel that debayering starts from.
-template <bool UNWIN = false, typename F>
+template <bool UNWIN = false, bool SLOT = false, typename F>
 __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (&t1)[8],
                                              const float (&t2)[8], const float (&t3)[8], int li,
                                              F &&emit) {
@@ -108,8 +112,13 @@ __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (
         const float A = fmaf(cr, t0[rho], sr * t2[rho]);
         const float B = fmaf(cr, t1[rho], sr * t3[rho]);
         const float wr = UNWIN ? 2.0f / kTab.hw[r < 8 ? r : 15 - r] : 0.5f * kTab.hw[r < 8 ? r : 15 - r];
-        emit(r, ca, wr * wca, fmaf(rha, B, A));
-        emit(r, cb, wr * wcb, fmaf(rhb, B, A));
+        if constexpr (SLOT) {
+            emit(r, ca, std::integral_constant<int, 0>(), wr * wca, fmaf(rha, B, A));
+            emit(r, cb, std::integral_constant<int, 1>(), wr * wcb, fmaf(rhb, B, A));
+        } else {
+            emit(r, ca, wr * wca, fmaf(rha, B, A));
+            emit(r, cb, wr * wcb, fmaf(rhb, B, A));
+        }
     }
 }
 
@@ -154,7 +163,11 @@ __global__ __launch_bounds__(64) void k_imclt_pass(Geo g, const float *clt, floa
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_IMCLT_WAVES)))
 void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
                                                      size_t dstride) {
-    __shared__ __align__(16) float acc[64][65];
+    // the block's sums (pitch AP), then 8 rows of discard space: with
+    // TP_IMCLT_REDIRECT a lane's outputs outside the block (whole 8-row halves
+    // of a column) are sent there, which costs no per-output test
+    constexpr int AP = 65, JUNK = 64 * AP;
+    __shared__ __align__(16) float acc[JUNK + 8 * AP];
     __shared__ __align__(16) float lds[4][8][TP_IMCLT_NQ][72];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int unit = lane >> 3, li = lane & 7;
@@ -170,7 +183,7 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
     const int bx = blk % nbx, by = blk / nbx;
     const int color = pidx % gridDim.y, cam = pidx / gridDim.y;
     const int X0 = bx * 64, Y0 = by * 64;
-    for (int i = threadIdx.x; i < 64 * 64; i += 256) acc[i >> 6][i & 63] = 0.f;
+    for (int i = threadIdx.x; i < 64 * AP; i += 256) acc[i] = 0.f;
     __syncthreads();
     const float *cl = clt[cam];
     for (int pass = 0; pass < 4; pass++) {
@@ -188,11 +201,29 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
             load_rows(cl + ((size_t)(tyc * g.TX + txc) * TP_NUM_COLORS + color) * 256, li, q0, q1, q2, q3);
             imclt_transform<TP_IMCLT_NQ>(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
             if (valid) {
-                const int oy = ty * 8 - Y0, ox = tx * 8 - X0;
-                imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {
-                    const int y = oy + r, x = ox + c;
-                    if ((unsigned)y < 64u && (unsigned)x < 64u) acc[y][x] = fmaf(w, d, acc[y][x]);
-                });
+                const int oy = ty * 8 - Y0, ox = tx * 8 - X0;  // -8 .. 56
+                if (TP_IMCLT_REDIRECT) {
+                    // the lane's 2 columns (as in imclt_unfold) x 2 row halves:
+                    // start of each in acc, or in the discard rows
+                    const int ca = li >= 4 ? li - 4 : 11 - li, cb = li >= 4 ? 11 - li : 12 + li;
+                    int base[2][2];
+#pragma unroll
+                    for (int cs = 0; cs < 2; cs++)
+#pragma unroll
+                        for (int hh = 0; hh < 2; hh++) {
+                            const int c = cs ? cb : ca, x = ox + c, y0 = oy + 8 * hh;
+                            base[cs][hh] = ((unsigned)x < 64u && (unsigned)y0 < 64u) ? y0 * AP + x : JUNK + c;
+                        }
+                    imclt_unfold<false, true>(q0, q1, q2, q3, li, [&](int r, int, auto cs, float w, float d) {
+                        float *p = acc + base[decltype(cs)::value][r >> 3] + (r & 7) * AP;
+                        *p = fmaf(w, d, *p);
+                    });
+                } else {
+                    imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {
+                        const int y = oy + r, x = ox + c;
+                        if ((unsigned)y < 64u && (unsigned)x < 64u) acc[y * AP + x] = fmaf(w, d, acc[y * AP + x]);
+                    });
+                }
             }
         }
         __syncthreads();
@@ -200,7 +231,7 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
     float *plane = rbg[cam] + dstride * (size_t)PH * color;
     for (int i = threadIdx.x; i < 64 * 64; i += 256) {
         const int y = Y0 + (i >> 6), x = X0 + (i & 63);
-        if (y < PH && x < PW) plane[(size_t)y * dstride + x] = acc[i >> 6][i & 63];
+        if (y < PH && x < PW) plane[(size_t)y * dstride + x] = acc[(i >> 6) * AP + (i & 63)];
     }
 }
 

@@ -74,6 +74,9 @@
 #ifndef TP_IMCLT_WAVES
 #define TP_IMCLT_WAVES 5      // whole-frame inverse CLT: minimum waves per SIMD
 #endif
+#ifndef TP_IMCLT_REDIRECT
+#define TP_IMCLT_REDIRECT 1   // whole-frame inverse CLT: out-of-block outputs to discard rows (no per-output test)
+#endif
 #ifndef TP_IMCLT_FAST
 #define TP_IMCLT_FAST 1       // inverse CLT: DCT-IV / DST-IV through a 4-point complex DFT (0: matrix form)
 #endif
