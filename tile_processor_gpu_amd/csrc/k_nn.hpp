@@ -518,13 +518,29 @@ __global__ __launch_bounds__(256) void k_nn_infer(const uint16_t *__restrict__ p
     }
     const float b30 = bf16f(prm[NN_OFF_B3]), b31 = bf16f(prm[NN_OFF_B3 + 1]);
     __syncthreads();
-    for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
+    // thread t's input elements t + 256 k of a tile, requested one tile ahead
+    constexpr int XPT = (NN_IN * NN_P + 255) / 256;
+    float xn[XPT];
+    auto load_x = [&](int tile) {
         const float *xt = x + (size_t)tile * NN_IN * NN_P;
-        for (int i = t; i < NN_IN * NN_P; i += 256) {
-            const int c = i / NN_P, p = i - c * NN_P;
-            X0[nn_padded(p) * 8 + c] = f2bf16(xt[i]);
+#pragma unroll
+        for (int k = 0; k < XPT; k++) {
+            const int i = t + 256 * k;
+            xn[k] = (tile < n && i < NN_IN * NN_P) ? xt[i] : 0.f;
+        }
+    };
+    load_x(blockIdx.x);
+    for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < XPT; k++) {
+            const int i = t + 256 * k;
+            if (i < NN_IN * NN_P) {
+                const int c = i / NN_P, p = i - c * NN_P;
+                X0[nn_padded(p) * 8 + c] = f2bf16(xn[k]);
+            }
         }
         __syncthreads();
+        load_x(tile + gridDim.x);
 #pragma unroll
         for (int bb = 0; bb < 2; bb++) {
             const int p = 32 * (2 * wave + bb) + co;
