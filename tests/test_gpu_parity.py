@@ -269,12 +269,14 @@ def test_combine_and_disparity_match_oracle(small, orc):
     assert np.abs(d[:, 0] - d_ref[:, 0]).max() < 1e-3
 
 
-def test_combine_irregular_list(small, orc):
+@pytest.mark.parametrize("r,stride", [(5, 130), (5, 121), (7, 225)])
+def test_combine_irregular_list(small, orc, r, stride):
     """Combine over a list that is not the full tile-major one: random pair
     subsets, tiles with duplicated entries (more than the 6 rows staged at a
-    time), tiles with no entry, radius 5 at a padded row stride.  Expected
-    output: per tile, the sum in list order of each entry's pair tile mapped
-    by the oracle's combination (one entry at a time)."""
+    time), tiles with no entry; a padded row stride, and contiguous rows
+    (stride (2r+1)^2, the software-pipelined path).  Expected output: per
+    tile, the sum in list order of each entry's pair tile mapped by the
+    oracle's combination (one entry at a time)."""
     g = small["g"]
     rng = np.random.default_rng(21)
     entries = []
@@ -288,7 +290,6 @@ def test_combine_irregular_list(small, orc):
             pairs = sorted(rng.choice(6, size=rng.integers(1, 7), replace=False).tolist())
         entries += [(t << 8) | p for p in pairs]
     idx = np.array(entries, np.int32)
-    r, stride = 5, 130
     n = 2 * r + 1
     fb = tp.FrameBuffers(g, "cuda:0", corr_radius=r, max_corr=len(idx), corr_stride=stride)
     for cam in range(4):
