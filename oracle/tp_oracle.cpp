@@ -1,7 +1,7 @@
 /*
  * tp_oracle.cpp - CPU oracle of the tile processor hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the correctness judge for the
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness reference for the
  * HIP kernels of libtp.so and the CPU baseline timed beside them.  Only the
  * tests, the smoke check and the benchmark's CPU-baseline leg may load it;
  * the product never calls it.

@@ -8,7 +8,8 @@ comes from scenes.render_frame: synthetic piecewise-planar scenes with known
 disparity, rendered to 4 Bayer cameras and passed through this pipeline
 (fused forward CLT + correlation, tp_convert_correlate).  The plain centroid
 of the combined tile (tp_corr_disparity, mode 1) is the baseline, the
-fallback for tiles the network is not confident about, and the judge.
+fallback for tiles the network is not confident about, and the yardstick
+its accuracy is compared with.
 
     python -m tile_processor_gpu_amd.nn train --tiles 2000000 --out weights.npz
 """
