@@ -85,6 +85,37 @@ def test_texture_tiles_match_oracle(tex, orc, keep, colors):
         assert fr >= 0.9999, (s, fr, worst)
 
 
+def test_texture_tiles_odd_list_and_skipped_entries(tex, orc):
+    """The kernel takes list entries two at a time: an odd-length list, and
+    entries without the texture bit or off the grid (skipped: their output
+    rows stay untouched), next to valid ones in the same pair."""
+    g = tex["g"]
+    idx = texture_indices(g, tex["tasks"])[:-3].copy()
+    assert len(idx) % 2 == 1
+    idx[5] &= ~(1 << 7)                          # no texture bit
+    idx[10] = (g.num_tiles + 3) << 8 | (1 << 7)  # off the grid
+    prm = orc.TexParams()
+    nsl = prm.slices()
+    stride = nsl * 256
+    ref = np.full((len(idx), stride), -9.0)
+    orc.textures_accumulate(g, None, tex["clt64"], idx, prm, tiles=ref, tex_stride=stride)
+    out = torch.full((len(idx), stride), -9.0, device="cuda:0")
+    kw = tp.TEXTURE_DEFAULTS
+    tp.textures_accumulate(g, None, tex["p_clt"], len(idx), torch.from_numpy(idx).cuda(), tex["ports"],
+                           kw["colors"], kw["is_lwir"], kw["min_shot"], kw["scale_shot"],
+                           kw["diff_sigma"], kw["diff_threshold"], kw["min_agree"], kw["weight0"],
+                           kw["weight1"], kw["weight2"], kw["dust_remove"], 0, 0, None, stride, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert (got[5] == -9.0).all() and (got[10] == -9.0).all()
+    assert (ref[5] == -9.0).all() and (ref[10] == -9.0).all()
+    keep = np.ones(len(idx), bool)
+    keep[[5, 10]] = False
+    for s in range(nsl):
+        fr, worst = frac_close(got[keep, s * 256:(s + 1) * 256], ref[keep, s * 256:(s + 1) * 256])
+        assert fr >= 0.9999, (s, fr, worst)
+
+
 def _gpu_generate(tex, colors=3, pad=8):
     g = tex["g"]
     t = np.ascontiguousarray(tex["tasks"])
