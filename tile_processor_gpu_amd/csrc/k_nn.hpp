@@ -577,11 +577,42 @@ __global__ __launch_bounds__(256) void k_nn_infer(const uint16_t *__restrict__ p
                 }
             }
         }
-        float prob, disp, cl;
-        head_stats(z0, z1, px, px ? t % 15 : 0, red, prob, disp, cl);
+        // softmax statistics of the head (as head_stats) with one barrier:
+        // each wavefront reduces relative to its own maximum, thread 0
+        // rescales the 4 partial sums to the common one
+        {
+            float m = px ? z0 : -INFINITY;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            const float e = px ? __expf(z0 - m) : 0.f;
+            float se = e, sx = e * (float)((px ? t % 15 : 0) - 7), sc = e * z1;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                se += __shfl_xor(se, o);
+                sx += __shfl_xor(sx, o);
+                sc += __shfl_xor(sc, o);
+            }
+            if (lane == 0) {
+                red[wave] = m;
+                red[4 + wave] = se;
+                red[8 + wave] = sx;
+                red[12 + wave] = sc;
+            }
+        }
+        __syncthreads();
         if (t == 0) {
-            out[2 * tile] = disp;
-            out[2 * tile + 1] = 1.0f / (1.0f + __expf(-cl));
+            const float M = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+            float S = 0.f, SX = 0.f, SC = 0.f;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                const float f = __expf(red[w] - M);  // 0 for a wavefront with no pixel
+                S = fmaf(red[4 + w], f, S);
+                SX = fmaf(red[8 + w], f, SX);
+                SC = fmaf(red[12 + w], f, SC);
+            }
+            const float inv = 1.0f / S;
+            out[2 * tile] = SX * inv;
+            out[2 * tile + 1] = 1.0f / (1.0f + __expf(-SC * inv));
         }
         __syncthreads();  // X0, H1, H2 and red are rewritten by the next tile
     }
