@@ -135,12 +135,13 @@ def _gpu_generate(tex, colors=3, pad=8):
     return tasks_out, ind.cpu().numpy(), num.cpu().numpy(), woi.cpu().numpy(), rbga.cpu().numpy(), stride
 
 
-@pytest.mark.parametrize("pad", [8, 3])  # 3: a row stride that rules out 16-byte stores
-def test_generate_rbga_matches_oracle(tex, orc, pad):
+# pad 3: a row stride that rules out 16-byte stores
+@pytest.mark.parametrize("pad,colors", [(8, 3), (3, 3), (8, 1)])
+def test_generate_rbga_matches_oracle(tex, orc, pad, colors):
     g = tex["g"]
-    tasks_out, ind, num, woi, rbga, stride = _gpu_generate(tex, pad=pad)
+    tasks_out, ind, num, woi, rbga, stride = _gpu_generate(tex, colors=colors, pad=pad)
     t_ref, woi_ref, num_ref, lists_ref, rbga_ref = orc.generate_rbga(g, tex["tasks"], tex["clt64"],
-                                                                     orc.TexParams(), stride)
+                                                                     orc.TexParams(colors=colors), stride)
     assert list(woi) == list(woi_ref)
     assert list(num) == list(num_ref)
     assert np.array_equal(tasks_out["task"], t_ref["task"])
@@ -148,9 +149,9 @@ def test_generate_rbga_matches_oracle(tex, orc, pad):
     for a, b in zip(lists, lists_ref):
         assert sorted(a.tolist()) == sorted(b.tolist())
     rows, cols = (woi[3] + 1) * 8, (woi[2] + 1) * 8
-    got = rbga[:4 * rows].reshape(4, rows, stride)
+    got = rbga[:(colors + 1) * rows].reshape(colors + 1, rows, stride)
     assert (got[:, :, cols:] == 7.0).all()  # outside the woi: untouched
-    for s in range(4):
+    for s in range(colors + 1):
         fr, worst = frac_close(got[s][:, :cols], rbga_ref[s][:, :cols])
         print(f"rgba slice {s}: close {fr:.6f}, worst {worst:.2e}")
         assert fr >= 0.9999, (s, fr, worst)

@@ -70,20 +70,21 @@ __device__ __forceinline__ float alpha_fade(int m, int i, int j) {
     }
 }
 
-// LDS of the workgroup's TP_TEX_TPW texture tiles.  U planes are TEX_MS
+// LDS of the workgroup's TPW texture tiles.  U planes are TEX_MS
 // floats apart: the imclt unfold writes columns {0-3, 8-11} (or {4-7,
 // 12-15}) of one row in each of 8 planes at once, and the 4-float skew
 // spreads those over all 32 banks.
 constexpr int TEX_MS = 260;
-constexpr int TEX_UNIT_WAVES = (TP_TEX_TPW * 12 + 7) / 8;  // wavefronts of phase 1 (3 colors)
 // Per-launch constants, computed once per workgroup: the pair factors
 // -log2(e) / (2 sigma^2 |port_j - port_i|^2) of the 6 port pairs, -log2(e) /
 // (2 sigma^2) and 1 / sqrt(min_shot).
 enum { TC_PAIR = 0, TC_KSIG = 6, TC_KSHOT = 7, TC_N = 8 };
+template <int TPW>
 struct TexShared {
-    float U[TP_TEX_TPW * 12 * TEX_MS];  // [tile][cam * colors + color][256], un-windowed
+    static constexpr int UNIT_WAVES = (TPW * 12 + 7) / 8;  // wavefronts of phase 1 (3 colors)
+    float U[TPW * 12 * TEX_MS];  // [tile][cam * colors + color][256], un-windowed
     float cst[TC_N];
-    float scratch[TEX_UNIT_WAVES][8][TP_TEX_NQ][72];  // phase 1: imclt transposes
+    float scratch[UNIT_WAVES][8][TP_TEX_NQ][72];  // phase 1: imclt transposes
 };
 
 // Per-pixel constants of thread t (pixel r = t / 16, c = t % 16), the same
@@ -289,19 +290,19 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
 }
 
-// List entries e0 .. e0 + TP_TEX_TPW - 1 (of n) by the whole workgroup:
+// List entries e0 .. e0 + TPW - 1 (of n) by the whole workgroup:
 // phase 1 runs the units of all its tiles at once (8 lanes each), then the
 // threads take the tiles' pixels one tile after the other.  Entries without
 // a texture bit, off the grid or past n are skipped (uniformly).
-template <int COLORS>
-__device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared &sh,
+template <int COLORS, int TPW>
+__device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared<TPW> &sh,
                                               const TexPix &px) {
     constexpr int colors = COLORS, UPT = 4 * COLORS;  // units per tile
     const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
-    int code[TP_TEX_TPW], tile[TP_TEX_TPW];
-    bool ok[TP_TEX_TPW], any = false;
+    int code[TPW], tile[TPW];
+    bool ok[TPW], any = false;
 #pragma unroll
-    for (int k = 0; k < TP_TEX_TPW; k++) {
+    for (int k = 0; k < TPW; k++) {
         code[k] = e0 + k < n ? list[e0 + k] : 0;
         tile[k] = code[k] >> TP_CORR_NTILE_SHIFT;
         ok[k] = (code[k] & (1 << TP_LIST_TEXTURE_BIT)) && tile[k] >= 0 && tile[k] < a.g.TX * a.g.TY;
@@ -312,14 +313,14 @@ __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, T
     const int slot = wave * 8 + (lane >> 3), li = lane & 7;
 
     // phase 1: U = imclt(CLT * lpf), un-windowed
-    if (slot < TP_TEX_TPW * UPT) {
+    if (slot < TPW * UPT) {
         const int k = slot / UPT, u = slot - k * UPT;
         const int cam = u / colors, col = u % colors;
         const int lw = colors > 1 ? col : 3;
         bool okk = ok[0];
         int tk = tile[0];
 #pragma unroll
-        for (int j = 1; j < TP_TEX_TPW; j++) {
+        for (int j = 1; j < TPW; j++) {
             okk = k == j ? ok[j] : okk;
             tk = k == j ? tile[j] : tk;
         }
@@ -338,22 +339,23 @@ __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, T
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < TP_TEX_TPW; k++)
+    for (int k = 0; k < TPW; k++)
         if (ok[k]) texture_pixel<COLORS>(a, e0 + k, code[k], tile[k], sh.U + k * UPT * TEX_MS, sh.cst, px);
 }
 
 // A fixed grid of workgroups loops over the list (its length may only be
-// known on the device), TP_TEX_TPW entries at a time.
-template <int COLORS>
+// known on the device), TPW entries at a time (2 for long lists; 1 for the
+// short border lists of generate_RBGA, whose passes are one round deep).
+template <int COLORS, int TPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
 void k_textures_accumulate(TexArgs a) {
-    __shared__ __align__(16) TexShared sh;
+    __shared__ __align__(16) TexShared<TPW> sh;
     const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
     const TexPix px = tex_pix(threadIdx.x);
     if (threadIdx.x == 0) tex_constants(a, sh.cst);
     __syncthreads();
-    for (int e = blockIdx.x * TP_TEX_TPW; e < n; e += gridDim.x * TP_TEX_TPW) {
-        texture_tiles<COLORS>(a, e, n, sh, px);
+    for (int e = blockIdx.x * TPW; e < n; e += gridDim.x * TPW) {
+        texture_tiles<COLORS, TPW>(a, e, n, sh, px);
         __syncthreads();  // the LDS of these entries is rewritten by the next ones
     }
 }

@@ -303,15 +303,24 @@ int tp_imclt_rbg_all(const tp_geometry *geom, float **gpu_clt, float **gpu_rbg, 
 
 namespace {
 
-int launch_textures(const tp::TexArgs &a, int grid, hipStream_t st) {
-    if (grid <= 0) return TP_OK;
+// n: list length or its cap; short: a list known to be short (one entry
+// per workgroup)
+int launch_textures(const tp::TexArgs &a, int n, hipStream_t st, bool short_list = false) {
+    if (n <= 0) return TP_OK;
     // workgroups loop over the list, TP_TEX_WG_PER_CU per CU
     const int cus = num_cus();
-    grid = std::min(grid, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
-    if (a.colors == 3)
-        hipLaunchKernelGGL(tp::k_textures_accumulate<3>, dim3(grid), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL(tp::k_textures_accumulate<1>, dim3(grid), dim3(256), 0, st, a);
+    const int grid = std::min(n, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
+    if (short_list) {
+        if (a.colors == 3)
+            hipLaunchKernelGGL((tp::k_textures_accumulate<3, 1>), dim3(grid), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((tp::k_textures_accumulate<1, 1>), dim3(grid), dim3(256), 0, st, a);
+    } else {
+        if (a.colors == 3)
+            hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
+    }
     return check_launch("textures_accumulate");
 }
 
@@ -495,7 +504,7 @@ int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
         p.idx = gpu_texture_indices + (pass & 3) * quarter;
         p.border_quarter = border ? quarter : 0;
         p.n = quarter;
-        if ((rc = launch_textures(p, quarter, st))) return rc;
+        if ((rc = launch_textures(p, quarter, st, border != 0))) return rc;
     }
     return TP_OK;
 }
