@@ -249,17 +249,49 @@ def debayer_shot(src, rb_mode, min_shot=10.0, scale_shot=3.0):
 
 
 def combine_pixel(D, M, r, c, colors=3, diff_sigma=1.5, weights=(0.294118, 0.117647, 0.588235),
-                  dust_remove=1, keep_weights=1, ports=PORTS):
+                  dust_remove=1, keep_weights=1, ports=PORTS, alt=0, eta=None):
+    """tile_combine_rgba of one pixel: out[colors + 1 + 4 + colors + 1].
+    alt overrides the discrete choices (bit 0: blend with the third-best
+    port, bit 1: the second-lowest port is the worst); with eta, returns
+    (out, ties[2]) where ties are the tie ratios of those two choices."""
     D = np.zeros((4, 3)) + np.asarray(D, np.float64)
     M = np.zeros((4, 3)) + np.asarray(M, np.float64)
     out = np.zeros(12)
+    ties = np.zeros(2)
     po = np.ascontiguousarray(ports, np.float32)
     f = ctypes.c_float
     lib().oracle_combine_pixel_f64(ctypes.c_int(colors), f(diff_sigma), f(weights[0]), f(weights[1]),
                                    f(weights[2]), ctypes.c_int(dust_remove), ctypes.c_int(keep_weights),
                                    po.ctypes.data_as(_VP), D.ctypes.data_as(_VP), M.ctypes.data_as(_VP),
-                                   ctypes.c_int(r), ctypes.c_int(c), out.ctypes.data_as(_VP))
-    return out
+                                   ctypes.c_int(r), ctypes.c_int(c), out.ctypes.data_as(_VP),
+                                   ctypes.c_int(alt), ctypes.c_double(eta or 0.0),
+                                   _VP(ties.ctypes.data if eta is not None else 0))
+    return (out, ties) if eta is not None else out
+
+
+def texture_ties(geom, clt, idx, params, alt=0, eta=1e-6, ports=PORTS):
+    """Per valid list entry: all slices [n][colors + 1 + 4 + colors + 1][256]
+    of its texture tile (FP64) with the consensus choices overridden by alt,
+    and the tie ratios [n][2][256] of every pixel (combine_pixel); rows of
+    invalid entries are NaN."""
+    clt = [np.ascontiguousarray(c, np.float64) for c in clt]
+    idx = np.ascontiguousarray(idx, np.int32)
+    nsl = 2 * params.colors + 6
+    sl = np.full((len(idx), nsl, 256), np.nan)
+    ties = np.full((len(idx), 2, 256), np.nan)
+    po = np.ascontiguousarray(ports, np.float32)
+    f = ctypes.c_float
+    p = params
+    g = _cgeo(geom)
+    lib().oracle_texture_ties_f64(ctypes.byref(g), _ptrs(clt), ctypes.c_size_t(len(idx)),
+                                  idx.ctypes.data_as(_VP), po.ctypes.data_as(_VP),
+                                  ctypes.c_int(p.colors), f(p.min_shot), f(p.scale_shot),
+                                  f(p.diff_sigma), f(p.diff_threshold), f(p.min_agree),
+                                  f(p.weights[0]), f(p.weights[1]), f(p.weights[2]),
+                                  ctypes.c_int(p.dust_remove), ctypes.c_int(alt),
+                                  ctypes.c_double(eta), sl.ctypes.data_as(_VP),
+                                  ties.ctypes.data_as(_VP))
+    return sl, ties
 
 
 def corr_combine(corrs, radius, pair_mask=0x3F):

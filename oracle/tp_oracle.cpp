@@ -642,10 +642,26 @@ void debayer_shot(int rb_mode, float min_shot, float scale_shot, const T src[16]
 }
 
 /* tile_combine_rgba for one pixel.  D[cam][color]: debayered, M: not
- * debayered.  out: colors values, alpha, 4 port weights, colors+1 rms. */
+ * debayered.  out: colors values, alpha, 4 port weights, colors+1 rms.
+ *
+ * The consensus makes two discrete choices: the two best ports of the
+ * pairwise weights (:4359-4376) and, with dust_remove, the worst port of the
+ * re-computed weights (:4463-4472).  For the checker they can be overridden
+ * and their robustness reported:
+ *   alt bit 0: blend the best port with the THIRD best instead of the second;
+ *   alt bit 1: treat the second-lowest port as the worst;
+ *   ties (if not NULL): ties[0] = (w_2nd - w_3rd) / (dw_2nd + dw_3rd) of the
+ *     pairwise port weights, ties[1] = (w_2nd_lowest - w_lowest) / (dw + dw)
+ *     of the re-computed ones (+inf without dust_remove), where dw bounds how
+ *     far a weight moves when every D value moves by eta * max|D| of the
+ *     pixel (first order in the colour differences plus their square, then
+ *     exactly through the exponential) plus eta of the weight itself.  A
+ *     ratio <= 1 means an implementation whose D values and arithmetic are
+ *     accurate only to eta may make the other choice at this pixel. */
 template <typename T>
 void combine_pixel(const TexParams &p, const float port_offsets[4][2], const T D[4][3],
-                   const T M[4][3], int r, int c, T *out) {
+                   const T M[4][3], int r, int c, T *out, int alt = 0, double eta = 0.0,
+                   double *ties = nullptr) {
     const Tables<T> &tb = tables<T>();
     const int colors = p.colors;
     /* pairs in the reference's (i < j) loop order */
@@ -683,16 +699,31 @@ void combine_pixel(const TexParams &p, const float port_offsets[4][2], const T D
     }
     const T wnd2 = tb.hwsq[sym16(r)] * tb.hwsq[sym16(c)];
     const T wnd2_inv = (T)1 / wnd2;
+    /* perturbation bound of every D value (ties only) */
+    double dD = 0.0, dpw[4] = {0, 0, 0, 0};
+    if (ties) {
+        for (int cam = 0; cam < 4; cam++)
+            for (int col = 0; col < colors; col++) dD = std::max(dD, std::fabs((double)D[cam][col]));
+        dD *= eta;
+    }
+    const double wi = (double)wnd2_inv;
     for (int cam = 0; cam < 4; cam++) pw[cam] = 0;
     for (int ip = 0; ip < 6; ip++) {
         T d = 0;
+        double dd = 0.0;
         for (int col = 0; col < colors; col++) {
             T dc = (D[pp[ip][0]][col] - D[pp[ip][1]][col]) * wnd2_inv;
             d += (T)p.weights[col] * dc * dc;
+            const double x = std::fabs((double)dc), dx = 2.0 * dD * wi;
+            dd += (double)p.weights[col] * (2.0 * x * dx + dx * dx);
         }
-        d = std::exp(-dist2r[ip] * d) + (T)TP_FAT_ZERO_WEIGHT;
+        const T e = std::exp(-dist2r[ip] * d);
+        d = e + (T)TP_FAT_ZERO_WEIGHT;
         pw[pp[ip][0]] += d;
         pw[pp[ip][1]] += d;
+        const double de = (double)e * std::expm1((double)dist2r[ip] * dd);
+        dpw[pp[ip][0]] += de;
+        dpw[pp[ip][1]] += de;
     }
     int b1 = 0;
     T best = pw[0];
@@ -702,22 +733,54 @@ void combine_pixel(const TexParams &p, const float port_offsets[4][2], const T D
     best = pw[b2];
     for (int cam = b2 + 1; cam < 4; cam++)
         if (cam != b1 && pw[cam] > best) { b2 = cam; best = pw[cam]; }
+    /* the third best: the better of the two remaining ports */
+    int b3 = -1;
+    for (int cam = 0; cam < 4; cam++)
+        if (cam != b1 && cam != b2 && (b3 < 0 || pw[cam] > pw[b3])) b3 = cam;
+    if (ties) {
+        for (int cam = 0; cam < 4; cam++) dpw[cam] += eta * (double)pw[cam];
+        const double m = (double)pw[b2] - (double)pw[b3], s = dpw[b2] + dpw[b3];
+        ties[0] = s > 0 ? m / s : (m > 0 ? INFINITY : 0.0);
+        ties[1] = INFINITY;
+    }
+    if (alt & 1) b2 = b3;
     const T w1 = pw[b1] / (pw[b1] + pw[b2]);
     const T w2 = (T)1 - w1;
     for (int col = 0; col < colors; col++) rgba[col] = w1 * D[b1][col] + w2 * D[b2][col];
+    /* perturbation of the blend: its inputs and its weight w1 */
+    const double sp = (double)pw[b1] + (double)pw[b2];
+    const double dw1 = ((double)pw[b2] * dpw[b1] + (double)pw[b1] * dpw[b2]) / (sp * sp);
+    double dq[4] = {0, 0, 0, 0};
     for (int cam = 0; cam < 4; cam++) {
         T d2 = 0;
+        double dd = 0.0;
         for (int col = 0; col < colors; col++) {
             T dc = (D[cam][col] - rgba[col]) * wnd2_inv;
             d2 += (T)p.weights[col] * dc * dc;
+            const double db = dD + std::fabs((double)D[b1][col] - (double)D[b2][col]) * dw1;
+            const double x = std::fabs((double)dc), dx = (dD + db) * wi;
+            dd += (double)p.weights[col] * (2.0 * x * dx + dx * dx);
         }
-        pw[cam] = std::exp(-ksigma * d2) + (T)TP_FAT_ZERO_WEIGHT;
+        const T e = std::exp(-ksigma * d2);
+        pw[cam] = e + (T)TP_FAT_ZERO_WEIGHT;
+        dq[cam] = (double)e * std::expm1((double)ksigma * dd) + eta * (double)pw[cam];
     }
     if (p.dust_remove) {
         int worst = 0;
         T wv = pw[0];
         for (int cam = 1; cam < 4; cam++)
             if (pw[cam] < wv) { worst = cam; wv = pw[cam]; }
+        int next = -1;  /* the second lowest */
+        for (int cam = 0; cam < 4; cam++)
+            if (cam != worst && (next < 0 || pw[cam] < pw[next])) next = cam;
+        if (ties) {
+            const double m = (double)pw[next] - (double)pw[worst], s = dq[next] + dq[worst];
+            ties[1] = s > 0 ? m / s : (m > 0 ? INFINITY : 0.0);
+        }
+        if (alt & 2) {
+            worst = next;
+            wv = pw[worst];
+        }
         T avg = -wv;
         for (int cam = 0; cam < 4; cam++) avg += pw[cam];
         avg /= (T)3;
@@ -740,10 +803,12 @@ void combine_pixel(const TexParams &p, const float port_offsets[4][2], const T D
     *alpha = wnd2 * a / (T)4;
 }
 
-/* one texture tile: slices [colors + 1 (+ 4 + colors + 1)][16][16] */
+/* one texture tile: slices [colors + 1 (+ 4 + colors + 1)][16][16]; alt, eta
+ * and ties[2][256] as for combine_pixel */
 template <typename T>
 void texture_tile(const Geo &g, const T *const *clt, int tile, const TexParams &p,
-                  const float port_offsets[4][2], T *slices) {
+                  const float port_offsets[4][2], T *slices, int alt = 0, double eta = 0.0,
+                  double *ties = nullptr) {
     const int colors = p.colors;
     T M[4][3][16][16], D[4][3][16][16];
     T lb[64];
@@ -769,8 +834,13 @@ void texture_tile(const Geo &g, const T *const *clt, int tile, const TexParams &
                     mp[cam][col] = M[cam][col][r][c];
                 }
             for (int i = 0; i < 12; i++) out[i] = 0;
-            combine_pixel<T>(p, port_offsets, dp, mp, r, c, out);
+            double tt[2];
+            combine_pixel<T>(p, port_offsets, dp, mp, r, c, out, alt, eta, ties ? tt : nullptr);
             for (int s = 0; s < nsl; s++) slices[s * 256 + r * 16 + c] = out[s];
+            if (ties) {
+                ties[r * 16 + c] = tt[0];
+                ties[256 + r * 16 + c] = tt[1];
+            }
         }
 }
 
@@ -1076,6 +1146,31 @@ void oracle_textures_accumulate_f32(const tp_geometry *geom, const int *woi, con
                                rbga_stride, rbga, tex_stride, tiles);
 }
 
+/* For the checker: every valid entry's slices [colors + 1 + 4 + colors + 1]
+ * [256] (tile form, nothing overlap-added) with the consensus choices
+ * overridden by alt, and (ties != NULL) the tie ratios [2][256] of each
+ * pixel (combine_pixel).  Rows of invalid entries are not written. */
+void oracle_texture_ties_f64(const tp_geometry *geom, const double *const *clt, size_t n,
+                             const int *idx, const float *port_offsets, int colors, float min_shot,
+                             float scale_shot, float diff_sigma, float diff_threshold,
+                             float min_agree, float w0, float w1, float w2, int dust_remove,
+                             int alt, double eta, double *slices, double *ties) {
+    TexParams p = tex_params(colors, min_shot, scale_shot, diff_sigma, diff_threshold, min_agree,
+                             w0, w1, w2, dust_remove, 1);
+    const Geo g = geo_of(geom);
+    const int nsl = 2 * colors + 6;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long e = 0; e < (long)n; e++) {
+        const int code = idx[e];
+        if (!(code & (1 << TP_LIST_TEXTURE_BIT))) continue;
+        const int tile = code >> TP_CORR_NTILE_SHIFT;
+        if (tile < 0 || tile >= g.TX * g.TY) continue;
+        texture_tile<double>(g, clt, tile, p, (const float(*)[2])port_offsets,
+                             slices + (size_t)e * nsl * 256, alt, eta,
+                             ties ? ties + (size_t)e * 512 : nullptr);
+    }
+}
+
 void oracle_prepare_texture_list(const tp_geometry *geom, tp_task *tasks, int num_tiles,
                                  int *indices, int *num_tex, int *woi, int width, int height) {
     prepare_texture_list(geo_of(geom), tasks, num_tiles, indices, num_tex, woi, width, height);
@@ -1088,11 +1183,12 @@ void oracle_debayer_shot_f64(int rb_mode, float min_shot, float scale_shot, cons
 }
 void oracle_combine_pixel_f64(int colors, float diff_sigma, float w0, float w1, float w2,
                               int dust_remove, int keep_weights, const float *port_offsets,
-                              const double *D, const double *M, int r, int c, double *out) {
+                              const double *D, const double *M, int r, int c, double *out,
+                              int alt, double eta, double *ties) {
     TexParams p = tex_params(colors, 10.f, 3.f, diff_sigma, 10.f, 3.f, w0, w1, w2, dust_remove,
                              keep_weights);
     combine_pixel<double>(p, (const float(*)[2])port_offsets, (const double(*)[3])D,
-                          (const double(*)[3])M, r, c, out);
+                          (const double(*)[3])M, r, c, out, alt, eta, ties);
 }
 
 /* combine: corrs rows of the 6 pairs of each tile (row index = tile * 6 +

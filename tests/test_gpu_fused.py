@@ -247,3 +247,83 @@ def test_stream_combined(stream):
     torch.cuda.synchronize()
     for i, fb in enumerate(fbs):
         assert rel_err(fb.combined[:fb.num_tiles].cpu().numpy(), refs[i].cpu().numpy()) <= TOL_STAGED
+
+
+@pytest.fixture(scope="module")
+def stream64(orc):
+    """The 64-frame sparse stream (clustered masks, 10-100 % of the tiles
+    active, per-frame images, kernels and shifts; the bench's batch) and the
+    FP64 oracle's correlation rows of every frame."""
+    frames = synth.config_stream(nframes=64, seed=4)
+    act = [f["active"] for f in frames]
+    print(f"[gpu fused] C4 stream: 64 frames, active {min(act):.3f}..{max(act):.3f}, "
+          f"{sum(len(f['tasks']) for f in frames)} tiles", flush=True)
+    fbs = [_frame(f) for f in frames]
+    refs = []
+    for f in frames:
+        g = f["geom"]
+        clt64 = orc.convert_correct_tiles(g, f["koffs"], f["kernels"], f["images"], f["tasks"])
+        refs.append(orc.correlate2D(g, clt64, 3, SCALES, FZ, corr_indices(g, f["tasks"]), 7))
+    return dict(frames=frames, fbs=fbs, refs=refs)
+
+
+@pytest.mark.parametrize("dct", [0, 1])
+def test_stream64_matches_oracle(stream64, dct):
+    """Every row (6 task + pair) of every frame of the 64-frame stream, run
+    as one persistent launch replayed from a captured graph (FP32 forward DCT,
+    or the FP16 hi + lo matrix-core form), within 1e-5 of the FP64 oracle;
+    rows past each frame's tasks untouched."""
+    fbs = stream64["fbs"]
+    for fb in fbs:
+        fb.corrs.fill_(SENTINEL)
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=8, dct=dct))
+    run.capture()
+    run.replay()
+    torch.cuda.synchronize()
+    assert run.drained()
+    worst = []
+    for i, (f, fb) in enumerate(zip(stream64["frames"], fbs)):
+        rows = _dense_rows(f)
+        allr = fb.corrs.cpu().numpy()
+        got = allr[rows]
+        e = rel_err(got, stream64["refs"][i])
+        worst.append(e)
+        assert e <= TOL, (dct, i, f["active"], e)
+        assert (allr[6 * fb.num_tiles:] == SENTINEL).all(), (dct, i)
+    i_min = int(np.argmin([f["active"] for f in stream64["frames"]]))
+    i_max = int(np.argmax([f["active"] for f in stream64["frames"]]))
+    print(f"dct {dct}: max rel err {max(worst):.2e}; sparsest frame ({stream64['frames'][i_min]['active']:.2f}) "
+          f"{worst[i_min]:.2e}, densest ({stream64['frames'][i_max]['active']:.2f}) {worst[i_max]:.2e}")
+    run.release()
+
+
+def test_stream64_new_task_lists_between_replays(stream64, orc):
+    """New sparse task lists uploaded into a frame after the graph was
+    captured (another mask, so another task count): the replay follows it,
+    and the frame's rows match the FP64 oracle on its new list."""
+    fbs = stream64["fbs"][:6]
+    frames = stream64["frames"][:6]
+    run = tp.StreamRun(fbs, 3, SCALES, FZ, config=dict(tiles_per_item=8))
+    run.capture()
+    rng = np.random.default_rng(17)
+    k = 2
+    f = frames[k]
+    g = f["geom"]
+    from tile_processor_gpu_amd.geometry import full_tasks
+    full = full_tasks(g, synth.tile_centres(g, 0.0, 0.5, rng))
+    # half the frame's own active fraction: the rows fit its corrs buffer
+    new_tasks = synth.sparse_tasks(full, synth.cluster_mask(g, rng, 0.5 * f["active"]))
+    assert 0 < len(new_tasks) < len(f["tasks"])
+    fbs[k].upload(tasks=new_tasks)
+    fbs[k].corrs.fill_(SENTINEL)
+    run.replay()
+    torch.cuda.synchronize()
+    clt64 = orc.convert_correct_tiles(g, f["koffs"], f["kernels"], f["images"], new_tasks)
+    ref = orc.correlate2D(g, clt64, 3, SCALES, FZ, corr_indices(g, new_tasks), 7)
+    nf = dict(f, tasks=new_tasks)
+    allr = fbs[k].corrs.cpu().numpy()
+    assert rel_err(allr[_dense_rows(nf)], ref) <= TOL
+    assert (allr[6 * len(new_tasks):] == SENTINEL).all()
+    # restore the fixture's frame
+    fbs[k].upload(tasks=f["tasks"])
+    run.release()

@@ -1,36 +1,23 @@
 """GPU parity of the texture path (textures_accumulate, prepare_texture_list,
 generate_RBGA) against the oracle.
 
-The 4-camera consensus makes discrete choices (best two ports, worst port);
-where two candidates' weights tie to within rounding, FP32 and FP64 may
-choose differently and that pixel differs.  The tests therefore require
-max |x - x64| / max |x64| <= 1e-5 on at least 99.99 % of the pixels (the
-rest are such ties) and report the worst pixel.
+Every output pixel must be within 1e-5 (max |x - x64| / max |x64| per slice)
+of the FP64 oracle, or be a pixel where the 4-camera consensus makes a
+choice that the oracle certifies as a tie at FP32 accuracy and within 1e-5
+of the FP64 output with that choice overridden (RGBA image: within 1e-5
+plus the change such flips of the overlapping tiles can make).  See
+tests/texcert.py.
 """
 import numpy as np
 import pytest
 import torch
 
-from tile_processor_gpu_amd import synth, tp
-from tile_processor_gpu_amd.geometry import Geometry, full_tasks, texture_indices, TASK_DTYPE
-from conftest import rel_err
+from tile_processor_gpu_amd import tp
+from tile_processor_gpu_amd.geometry import Geometry, texture_indices, TASK_DTYPE
+from texcert import scene_cfg, certify_tiles, rbga_slack, certify_rbga
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-5
 PORTS = np.array([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]], np.float32)
-
-
-def scene_cfg(seed, geom):
-    """4 cameras of one scene (raw-pixel scale, 20..220) with independent
-    noise, identity-ish PSF kernels and sub-pixel jitter."""
-    rng = np.random.default_rng(seed)
-    from scipy.ndimage import gaussian_filter
-    base = gaussian_filter(rng.standard_normal((geom.img_height, geom.img_width)), 2.0)
-    base = 20 + 200 * (base - base.min()) / (base.max() - base.min())
-    imgs = np.stack([base + rng.normal(0, 1.0, base.shape) for _ in range(4)]).astype(np.float32)
-    kern, koffs = synth.make_kernels(geom, rng, sigma=(0.5, 1.0))
-    tasks = full_tasks(geom, synth.tile_centres(geom, 0.0, 0.25, rng))
-    return imgs, kern, koffs, tasks
 
 
 @pytest.fixture(scope="module")
@@ -44,12 +31,6 @@ def tex(orc):
                 ports=torch.from_numpy(PORTS).cuda())
 
 
-def frac_close(got, ref, tol=TOL):
-    den = np.abs(ref).max()
-    bad = np.abs(got - ref) > tol * den
-    return 1.0 - bad.mean(), float(np.abs(got - ref).max() / den)
-
-
 @pytest.mark.parametrize("colors", [3, 1])
 @pytest.mark.parametrize("keep", [0, 1])
 def test_texture_tiles_match_oracle(tex, orc, keep, colors):
@@ -58,8 +39,6 @@ def test_texture_tiles_match_oracle(tex, orc, keep, colors):
     prm = orc.TexParams(keep_weights=keep, colors=colors)
     nsl = prm.slices()
     stride = nsl * 256 + 64
-    ref = np.zeros((len(idx), stride))
-    orc.textures_accumulate(g, None, tex["clt64"], idx, prm, tiles=ref, tex_stride=stride)
     d_idx = torch.from_numpy(idx).cuda()
     out = torch.full((len(idx), stride), -9.0, device="cuda:0")
     kw = dict(tp.TEXTURE_DEFAULTS, keep_weights=keep, colors=colors)
@@ -70,19 +49,9 @@ def test_texture_tiles_match_oracle(tex, orc, keep, colors):
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     assert (got[:, nsl * 256:] == -9.0).all()
-    # port weights (slices colors + 1 .. colors + 4) enter the output only
-    # through alpha = w^2 sum(weights) / 4; at tile corners they are
-    # ill-conditioned by construction (colour differences are scaled by
-    # 1 / w^4 before the exponential), so they are compared as w^2 * weight
-    w = np.sin(np.pi * (np.arange(16) + 0.5) / 16) ** 2
-    w2 = np.outer(w, w).ravel()
-    for s in range(nsl):
-        a, b = got[:, s * 256:(s + 1) * 256], ref[:, s * 256:(s + 1) * 256]
-        if colors + 1 <= s < colors + 5:
-            a, b = a * w2, b * w2
-        fr, worst = frac_close(a, b)
-        print(f"slice {s}: close {fr:.6f}, worst {worst:.2e}")
-        assert fr >= 0.9999, (s, fr, worst)
+    n_bad, n_tied = certify_tiles(orc, g, tex["clt64"], idx, prm,
+                                  got[:, :nsl * 256].reshape(len(idx), nsl, 256), nsl)
+    print(f"{n_bad} pixels beyond 1e-5, all certified ties ({n_tied} tie pixels)")
 
 
 def test_texture_tiles_odd_list_and_skipped_entries(tex, orc):
@@ -97,8 +66,6 @@ def test_texture_tiles_odd_list_and_skipped_entries(tex, orc):
     prm = orc.TexParams()
     nsl = prm.slices()
     stride = nsl * 256
-    ref = np.full((len(idx), stride), -9.0)
-    orc.textures_accumulate(g, None, tex["clt64"], idx, prm, tiles=ref, tex_stride=stride)
     out = torch.full((len(idx), stride), -9.0, device="cuda:0")
     kw = tp.TEXTURE_DEFAULTS
     tp.textures_accumulate(g, None, tex["p_clt"], len(idx), torch.from_numpy(idx).cuda(), tex["ports"],
@@ -108,12 +75,8 @@ def test_texture_tiles_odd_list_and_skipped_entries(tex, orc):
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     assert (got[5] == -9.0).all() and (got[10] == -9.0).all()
-    assert (ref[5] == -9.0).all() and (ref[10] == -9.0).all()
-    keep = np.ones(len(idx), bool)
-    keep[[5, 10]] = False
-    for s in range(nsl):
-        fr, worst = frac_close(got[keep, s * 256:(s + 1) * 256], ref[keep, s * 256:(s + 1) * 256])
-        assert fr >= 0.9999, (s, fr, worst)
+    # the oracle skips the same two entries (their rows are ignored here)
+    certify_tiles(orc, g, tex["clt64"], idx, prm, got.reshape(len(idx), nsl, 256), nsl)
 
 
 def _gpu_generate(tex, colors=3, pad=8):
@@ -151,10 +114,10 @@ def test_generate_rbga_matches_oracle(tex, orc, pad, colors):
     rows, cols = (woi[3] + 1) * 8, (woi[2] + 1) * 8
     got = rbga[:(colors + 1) * rows].reshape(colors + 1, rows, stride)
     assert (got[:, :, cols:] == 7.0).all()  # outside the woi: untouched
-    for s in range(colors + 1):
-        fr, worst = frac_close(got[s][:, :cols], rbga_ref[s][:, :cols])
-        print(f"rgba slice {s}: close {fr:.6f}, worst {worst:.2e}")
-        assert fr >= 0.9999, (s, fr, worst)
+    ref = rbga_ref[:, :rows, :cols]
+    slack = rbga_slack(orc, g, tex["clt64"], lists_ref, woi_ref, orc.TexParams(colors=colors), ref.shape)
+    n = certify_rbga(got[:, :, :cols], ref, slack)
+    print(f"{n} RGBA pixels beyond 1e-5, all within the certified-tie slack")
 
 
 def test_prepare_texture_list_sparse(tex, orc):

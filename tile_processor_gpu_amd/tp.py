@@ -153,6 +153,28 @@ def wall_clock_hz():
     return _lib.lib().tp_wall_clock_hz()
 
 
+def frame_descriptors(frames, store_clt=False, store_pairs=True, combined=False):
+    """tp_frame array of the stream form for a list of FrameBuffers, after
+    checking that each frame has a task list and that its outputs hold the
+    rows the kernel writes: 6 rows per task (row 6 task + pair) for the pair
+    tiles and one per task for the six-pair combination."""
+    desc = np.zeros(len(frames), dtype=FRAME_DTYPE)
+    for i, fb in enumerate(frames):
+        if fb.tasks is None:
+            raise ValueError(f"frame {i}: no task list uploaded")
+        if store_pairs and fb.corrs.shape[0] < 6 * fb.num_tiles:
+            raise ValueError(f"frame {i}: corrs has {fb.corrs.shape[0]} rows, the stream form "
+                             f"writes 6 per task ({6 * fb.num_tiles})")
+        comb = fb.combined_buffer() if combined else None
+        if comb is not None and comb.shape[0] < fb.num_tiles:
+            raise ValueError(f"frame {i}: combined buffer has fewer rows than tasks")
+        desc[i] = (fb.tasks.data_ptr(), fb.p_images.data_ptr(), fb.p_kernels.data_ptr(),
+                   fb.p_koffs.data_ptr(), fb.p_clt.data_ptr() if store_clt else 0,
+                   fb.corrs.data_ptr() if store_pairs else 0,
+                   comb.data_ptr() if comb is not None else 0, fb.num_tiles, 0)
+    return desc
+
+
 class StreamRun:
     """The stream form over a list of FrameBuffers: one persistent launch
     (tp_stream_convert_correlate) per call, or a captured graph of it."""
@@ -182,19 +204,27 @@ class StreamRun:
         self.args = (colors, *scales, fat_zero)
         self.lpf_mask = lpf_mask
         self.config = FusedConfig(**config) if isinstance(config, dict) else (config or FusedConfig())
-        desc = np.zeros(len(frames), dtype=FRAME_DTYPE)
-        for i, fb in enumerate(frames):
-            desc[i] = (fb.tasks.data_ptr(), fb.p_images.data_ptr(), fb.p_kernels.data_ptr(),
-                       fb.p_koffs.data_ptr(), fb.p_clt.data_ptr() if store_clt else 0,
-                       fb.corrs.data_ptr() if store_pairs else 0,
-                       fb.combined_buffer().data_ptr() if combined else 0, fb.num_tiles, 0)
+        self.store_clt, self.store_pairs, self.combined = store_clt, store_pairs, combined
         self.pair_mask = pair_mask
-        self.desc = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+        self._desc_host = frame_descriptors(frames, store_clt, store_pairs, combined)
+        self.desc = torch.from_numpy(self._desc_host.view(np.uint8).copy()).to(dev)
         # per-XCD queue counters (one 128-byte line each) and the item count
         self.work = torch.zeros(stream_work_ints(len(frames)), dtype=torch.int32, device=dev)
         # per-frame GPU time stamps (two atomics per work item on the frame's words)
         self.stamps = torch.zeros((len(frames), 2), dtype=torch.int64, device=dev) if stamps else None
         self.graph = None
+
+    def refresh(self):
+        """Re-check the frames and rewrite the device frame descriptors in
+        place if a frame's buffers or task count changed since the last
+        launch (a captured graph reads the descriptors at replay time, so it
+        follows).  Raises ValueError if a frame's output rows no longer fit."""
+        d = frame_descriptors(self.frames, self.store_clt, self.store_pairs, self.combined)
+        if d.tobytes() != self._desc_host.tobytes():
+            self.desc.copy_(torch.from_numpy(d.view(np.uint8).copy()))
+            self._desc_host = d
+            return True
+        return False
 
     def _common(self):
         f0 = self.frames[0]
@@ -205,10 +235,12 @@ class StreamRun:
                 int(f0.combined_stride), int(self.pair_mask), ctypes.byref(self.config))
 
     def launch(self, stream=None):
+        self.refresh()
         _lib.call("tp_stream_convert_correlate", *self._common(), _stream(stream))
 
     def capture(self):
         self.release()
+        self.refresh()
         h = _VP(0)
         _lib.call("tp_stream_graph_create", *self._common(), ctypes.byref(h))
         self.graph = h
@@ -216,6 +248,7 @@ class StreamRun:
     def replay(self, stream=None):
         if self.graph is None:
             self.capture()
+        self.refresh()
         _lib.call("tp_graph_launch", self.graph, _stream(stream))
 
     def release(self):
@@ -288,8 +321,16 @@ class FrameBuffers:
                 self.koffs[c].copy_(torch.from_numpy(
                     np.ascontiguousarray(koffs[c], dtype=np.float32).reshape(-1)))
         if tasks is not None:
+            # one task buffer per frame, sized for a full task list; task
+            # lists are copied into it, so its address (which a stream's
+            # frame descriptors hold) stays the same while they fit
             t = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
-            self.tasks = torch.from_numpy(t.view(np.uint8).copy()).to(self.device)
+            nbytes = t.nbytes
+            if self.tasks is None or self.tasks.numel() < nbytes:
+                cap = max(nbytes, self.geom.num_tiles * TASK_DTYPE.itemsize)
+                self.tasks = torch.zeros(cap, dtype=torch.uint8, device=self.device)
+            if nbytes:
+                self.tasks[:nbytes].copy_(torch.from_numpy(t.view(np.uint8).copy()))
             self.num_tiles = len(t)
         if corr_idx is not None:
             ci = np.ascontiguousarray(corr_idx, dtype=np.int32)
@@ -311,8 +352,8 @@ class FrameBuffers:
         """[tiles][(2r+1)^2] six-pair combination rows (row = task), made on first use."""
         n = 2 * self.corr_radius + 1
         if getattr(self, "combined", None) is None or self.combined.shape[0] < max(1, self.num_tiles):
-            self.combined = torch.zeros((max(1, self.num_tiles), n * n), dtype=torch.float32,
-                                        device=self.device)
+            rows = max(1, self.num_tiles, self.geom.num_tiles)
+            self.combined = torch.zeros((rows, n * n), dtype=torch.float32, device=self.device)
         return self.combined
 
     @property
