@@ -492,6 +492,18 @@ int tp_nn_adam(float *gpu_params, float *gpu_m, float *gpu_v, const float *gpu_g
 /* BF16 copy (round to nearest even) of n FP32 values. */
 int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t stream);
 
+/* 2-D 8x8 transform of every 8x8 tile of a width x height image, in place
+ * of the tile (row = vertical frequency, column = horizontal frequency).
+ * Replaces GPU_DTT24_DRV (src/dtt8x8.cuh:103-161), the reference's test
+ * entry for its DCT/DST butterflies; width and height replace the grid its
+ * caller launches (32 x 16 pixel blocks) and may be any multiples of 8.
+ *   dtt_mode: 0 horizontal DCT-IV, vertical DCT-IV; 1 DST-IV, DCT-IV;
+ *             2 DCT-IV, DST-IV; 3 DST-IV, DST-IV; 4-7 the same with DCT-II /
+ *             DST-II (all orthonormal)
+ *   gpu_src, gpu_dst: float[height][src_stride] (may be the same buffer) */
+int tp_dtt8x8(float *gpu_dst, const float *gpu_src, int src_stride, int dtt_mode, int width,
+              int height, tp_stream_t stream);
+
 /* gpu_dst[0..n) = value, by a kernel (safe inside captured graphs). */
 int tp_fill(float *gpu_dst, size_t n, float value, tp_stream_t stream);
 

@@ -96,3 +96,14 @@ def test_hbm_probe_library():
     assert P.tp_probe_hbm(0, NZ, NZ, NZ, 1024, 0, None) == -1     # bad blocks per CU
     assert P.tp_probe_hbm(0, None, NZ, NZ, 1024, 4, None) == -1   # read without a source
     assert P.tp_probe_hbm(3, NZ, None, NZ, 1024, 4, None) == -1   # mix without a destination
+
+
+def test_dtt8x8_rejects_bad_arguments():
+    ok = (NZ, NZ, 64, 0, 64, 32, None)
+    assert rc_err("tp_dtt8x8", None, *ok[1:])[0] == -1
+    rc, msg = rc_err("tp_dtt8x8", NZ, NZ, 64, 8, 64, 32, None)
+    assert rc == -1 and "dtt_mode" in msg
+    rc, msg = rc_err("tp_dtt8x8", NZ, NZ, 64, 0, 60, 32, None)
+    assert rc == -1 and "multiples of 8" in msg
+    rc, msg = rc_err("tp_dtt8x8", NZ, NZ, 40, 0, 64, 32, None)
+    assert rc == -1 and "src_stride" in msg

@@ -18,6 +18,7 @@
 #include "k_combine.hpp"
 #include "k_fused.hpp"
 #include "k_nn.hpp"
+#include "k_dtt.hpp"
 
 namespace {
 
@@ -756,6 +757,20 @@ int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t 
     hipLaunchKernelGGL(tp::k_nn_pack, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), gpu_src, n,
                        gpu_dst);
     return check_launch("nn_pack_bf16");
+}
+
+int tp_dtt8x8(float *gpu_dst, const float *gpu_src, int src_stride, int dtt_mode, int width,
+              int height, tp_stream_t stream) {
+    if (!gpu_dst || !gpu_src) return fail(TP_EINVAL, "tp_dtt8x8: NULL pointer argument%s", "");
+    if (dtt_mode < 0 || dtt_mode > 7) return fail(TP_EINVAL, "tp_dtt8x8: dtt_mode must be 0..7%s", "");
+    if (width <= 0 || height <= 0 || (width % 8) || (height % 8))
+        return fail(TP_EINVAL, "tp_dtt8x8: width and height must be positive multiples of 8%s", "");
+    if (src_stride < width) return fail(TP_EINVAL, "tp_dtt8x8: src_stride < width%s", "");
+    g_err.clear();
+    const int tiles_x = width / 8, ntiles = tiles_x * (height / 8);
+    hipLaunchKernelGGL(tp::k_dtt8x8, dim3((ntiles + 31) / 32), dim3(256), 0, as_stream(stream), gpu_src,
+                       gpu_dst, (size_t)src_stride, tiles_x, ntiles, dtt_mode);
+    return check_launch("dtt8x8");
 }
 
 int tp_fill(float *gpu_dst, size_t n, float value, tp_stream_t stream) {

@@ -437,3 +437,12 @@ def corr_disparity(gpu_tiles, stride, num_tiles, corr_radius, mode, gpu_out, str
     """(disparity, strength) per combined tile; mode 0 parabola, 1 centroid."""
     _lib.call("tp_corr_disparity", _p(gpu_tiles), int(stride), int(num_tiles), int(corr_radius),
               int(mode), _p(gpu_out), _stream(stream))
+
+
+def dtt8x8(gpu_dst, gpu_src, src_stride, dtt_mode, width, height, stream=None):
+    """GPU_DTT24_DRV (src/dtt8x8.cuh:103-161): 2-D 8x8 DCT/DST of every 8x8
+    tile (mode bit 0: horizontal sine, bit 1: vertical sine, modes 4-7:
+    type II instead of IV; orthonormal)."""
+    _check_dev(gpu_dst, gpu_src)
+    _lib.call("tp_dtt8x8", _p(gpu_dst), _p(gpu_src), int(src_stride), int(dtt_mode), int(width),
+              int(height), _stream(stream))
