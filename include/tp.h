@@ -415,6 +415,35 @@ int tp_stream_convert_correlate(const tp_geometry *geom,
                                 const tp_fused_config *config,
                                 tp_stream_t stream);
 
+/* tp_stream_convert_correlate without the gpu_work argument.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, hosts that forbid
+ * it).  The work buffer is a fixed array of the library, one per GPU, sized
+ * for TP_STREAM_NOALLOC_MAX_FRAMES frames; larger nframes is refused with
+ * TP_EINVAL.  The call allocates nothing (the first call on a device looks
+ * up that device's array).  Every call on one GPU uses the same array:
+ * calls that overlap on one GPU (on different streams, or from several
+ * threads) race and must be serialised by the caller; consecutive calls on
+ * one stream are fine (each call re-initialises the array). */
+#define TP_STREAM_NOALLOC_MAX_FRAMES 4096
+int tp_stream_convert_correlate_noalloc(const tp_geometry *geom,
+                                        tp_frame *gpu_frames,
+                                        int nframes,
+                                        uint64_t *gpu_stamps,
+                                        size_t dstride,
+                                        int lpf_mask,
+                                        int colors,
+                                        float scale0,
+                                        float scale1,
+                                        float scale2,
+                                        float fat_zero,
+                                        size_t corr_stride,
+                                        int corr_radius,
+                                        size_t combined_stride,
+                                        int pair_mask,
+                                        const tp_fused_config *config,
+                                        tp_stream_t stream);
+
 /* The same stream launch captured once into an executable HIP graph
  * (set-up kernel + persistent kernel, no memset nodes); replaying it
  * re-reads every buffer, so new frame data can be uploaded between

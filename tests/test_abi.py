@@ -107,3 +107,13 @@ def test_dtt8x8_rejects_bad_arguments():
     assert rc == -1 and "multiples of 8" in msg
     rc, msg = rc_err("tp_dtt8x8", NZ, NZ, 40, 0, 64, 32, None)
     assert rc == -1 and "src_stride" in msg
+
+
+def test_stream_noalloc_rejects_too_many_frames():
+    g = Geometry(256, 256).c()
+    F = ctypes.c_float
+    args = lambda n: (ctypes.byref(g), NZ, n, None, 256, 0, 3, F(0.25), F(0.25), F(0.5),  # noqa: E731
+                      F(0.05), 225, 7, 225, 0x3F, None, None)
+    rc, msg = rc_err("tp_stream_convert_correlate_noalloc", *args(4097))
+    assert rc == -1 and "4096" in msg
+    assert rc_err("tp_stream_convert_correlate_noalloc", *args(0))[0] == -1

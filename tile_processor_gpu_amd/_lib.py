@@ -38,6 +38,8 @@ PROTOTYPES = {
                              _P, _SZ, _I, _P, _P],
     "tp_stream_convert_correlate": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _SZ,
                                     _I, _P, _P],
+    "tp_stream_convert_correlate_noalloc": [_G, _P, _I, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I,
+                                            _SZ, _I, _P, _P],
     "tp_stream_graph_create": [_G, _P, _I, _P, _P, _SZ, _I, _I, _F, _F, _F, _F, _SZ, _I, _SZ, _I,
                                _P, _P],
     "tp_graph_launch": [_P, _P],

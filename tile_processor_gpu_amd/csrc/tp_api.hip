@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "tp_common.hpp"
@@ -159,6 +160,45 @@ int enqueue_stream(const tp_geometry *geom, tp_frame *gpu_frames, int nframes, i
     if ((rc = check_launch("stream_setup"))) return rc;
     launch_fused<true>(a, c, colors, cus * c.bpc, st);
     return check_launch("fused (stream)");
+}
+
+}  // namespace
+
+// Work buffer of tp_stream_convert_correlate_noalloc: queue counters, item
+// count and per-queue item prefixes of up to TP_STREAM_NOALLOC_MAX_FRAMES
+// frames (132 KB), one copy per GPU for the life of the process.  In host
+// code this name is not a device address; host code uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_ptr).
+static __device__ __align__(256) int32_t scratch[TP_STREAM_WORK_INTS(TP_STREAM_NOALLOC_MAX_FRAMES)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device and kept.
+int scratch_ptr(const char *fn, int32_t **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(TP_EHIP, "%s: hipGetDevice failed", fn);
+    if (dev < 0 || dev >= kMaxDevices) return fail(TP_EHIP, "%s: device index out of range", fn);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            char buf[256];
+            snprintf(buf, sizeof(buf), "%s: hipGetSymbolAddress of the work buffer: %s", fn,
+                     hipGetErrorString(e));
+            g_err = buf;
+            return TP_EHIP;
+        }
+        g_scratch[dev] = (int32_t *)p;
+    }
+    *out = g_scratch[dev];
+    return TP_OK;
 }
 
 }  // namespace
@@ -605,6 +645,27 @@ int tp_stream_convert_correlate(const tp_geometry *geom, tp_frame *gpu_frames, i
     return enqueue_stream(geom, gpu_frames, nframes, gpu_work, gpu_stamps, dstride, lpf_mask, colors,
                           scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, combined_stride,
                           pair_mask, config, as_stream(stream), "tp_stream_convert_correlate");
+}
+
+int tp_stream_convert_correlate_noalloc(const tp_geometry *geom, tp_frame *gpu_frames, int nframes,
+                                        uint64_t *gpu_stamps, size_t dstride, int lpf_mask,
+                                        int colors, float scale0, float scale1, float scale2,
+                                        float fat_zero, size_t corr_stride, int corr_radius,
+                                        size_t combined_stride, int pair_mask,
+                                        const tp_fused_config *config, tp_stream_t stream) {
+    const char *fn = "tp_stream_convert_correlate_noalloc";
+    g_err.clear();
+    if (nframes > TP_STREAM_NOALLOC_MAX_FRAMES)
+        return fail(TP_EINVAL, "%s: nframes %ld exceeds the built-in work buffer (4096 frames)", fn,
+                    (long)nframes);
+    if (!gpu_frames) return fail(TP_EINVAL, "%s: NULL pointer argument", fn);
+    if (nframes <= 0) return fail(TP_EINVAL, "%s: nframes must be positive", fn);
+    int32_t *work = nullptr;
+    int rc = scratch_ptr(fn, &work);
+    if (rc) return rc;
+    return enqueue_stream(geom, gpu_frames, nframes, work, gpu_stamps, dstride, lpf_mask, colors,
+                          scale0, scale1, scale2, fat_zero, corr_stride, corr_radius, combined_stride,
+                          pair_mask, config, as_stream(stream), fn);
 }
 
 int tp_stream_graph_create(const tp_geometry *geom, tp_frame *gpu_frames, int nframes,

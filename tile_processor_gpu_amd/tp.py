@@ -238,6 +238,13 @@ class StreamRun:
         self.refresh()
         _lib.call("tp_stream_convert_correlate", *self._common(), _stream(stream))
 
+    def launch_noalloc(self, stream=None):
+        """The same launch through tp_stream_convert_correlate_noalloc (the
+        library's own work buffer instead of self.work)."""
+        self.refresh()
+        args = self._common()
+        _lib.call("tp_stream_convert_correlate_noalloc", *args[:3], *args[4:], _stream(stream))
+
     def capture(self):
         self.release()
         self.refresh()
