@@ -401,6 +401,33 @@ def measure_stages(cfg, fb, idx, reps):
     return out
 
 
+def setup_frame(seed):
+    """The full 4 x 2592 x 1936 frame in HBM.  Its aberration kernels and
+    CltExtra records are made on the GPU by tp_psf_to_clt from sampled
+    anisotropic Gaussian PSFs (15 x 15, sigma 0.5-2 px, random orientation,
+    centroid U[-0.25, 0.25] px), then read back so that the oracle and the CPU
+    baseline use the same kernels.  Returns (cfg, fb, idx, kernel_gen)."""
+    import torch
+    from tile_processor_gpu_amd import synth, tp
+    from tile_processor_gpu_amd.geometry import corr_indices
+    cfg = synth.config_full(seed=seed)
+    g = cfg["geom"]
+    idx = corr_indices(g, cfg["tasks"])
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
+    fb.upload(cfg["images"], None, None, cfg["tasks"], idx)
+    psfs = torch.from_numpy(synth.psf_grid(g, np.random.default_rng(seed + 1000))).cuda()
+    keep = fb.kernels_from_psfs(psfs)
+    ms = median_ms(lambda: fb.kernels_from_psfs(psfs), 5, 1)
+    torch.cuda.synchronize()
+    del keep
+    cfg["kernels"] = np.stack([k.cpu().numpy() for k in fb.kernels])
+    cfg["koffs"] = np.stack([k.cpu().numpy() for k in fb.koffs])
+    gen = {"ms": round(ms, 4), "psfs": int(psfs.shape[0] * psfs.shape[1] * psfs.shape[2] * psfs.shape[3]),
+           "psf_size": int(psfs.shape[-1])}
+    del psfs
+    return cfg, fb, idx, gen
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,21 +468,14 @@ def main():
         return
 
     if args.mode == "fused":  # the fused kernel alone (default launch config), for profiling
-        cfg = synth.config_full(seed=args.seed)
-        g = cfg["geom"]
-        idx = corr_indices(g, cfg["tasks"])
-        fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
-        fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+        cfg, fb, idx, _ = setup_frame(args.seed)
         ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO), max(1, args.steps), args.warmup)
         print(json.dumps({"mode": "fused", "ms": round(ms, 4), "steps": args.steps}), flush=True)
         return
 
     if args.mode == "texture":
-        cfg = synth.config_full(seed=args.seed)
+        cfg, fb, idx, _ = setup_frame(args.seed)
         g = cfg["geom"]
-        idx = corr_indices(g, cfg["tasks"])
-        fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
-        fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
         fb.convert()
         ms_tex = median_ms(make_texture(cfg, fb), max(1, args.steps), args.warmup)
         dstride = g.img_width + 8
@@ -468,13 +488,10 @@ def main():
         return
 
     t0 = time.time()
-    cfg = synth.config_full(seed=args.seed)
+    cfg, fb, idx, kernel_gen = setup_frame(args.seed)
     g = cfg["geom"]
-    idx = corr_indices(g, cfg["tasks"])
-    log(f"inputs {g.img_width}x{g.img_height}, {g.num_tiles} tiles, {len(idx)} pairs "
-        f"({time.time() - t0:.1f} s)")
-    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=RADIUS, max_corr=len(idx))
-    fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+    log(f"inputs {g.img_width}x{g.img_height}, {g.num_tiles} tiles, {len(idx)} pairs, "
+        f"kernels from {kernel_gen['psfs']} PSFs in {kernel_gen['ms']} ms ({time.time() - t0:.1f} s)")
     torch.cuda.synchronize()
 
     def step():
@@ -573,7 +590,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Bayer frames, Gaussian-PSF kernels, planar disparity 0-20 px)",
+        "data": "synthetic (seeded Bayer frames; aberration kernels made on the GPU from sampled "
+                "anisotropic Gaussian PSFs; planar disparity 0-20 px)",
         "config": {"workload": "plain FP32 staged CLT + aberration + 6-pair correlation (r=7)",
                    "cameras": 4, "width": g.img_width, "height": g.img_height,
                    "tiles": g.num_tiles, "pairs": int(len(idx)), "fat_zero": FAT_ZERO,
@@ -584,6 +602,7 @@ def main():
         "stages": stage,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
+        "kernel_generation": kernel_gen,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

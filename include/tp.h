@@ -521,6 +521,31 @@ int tp_nn_adam(float *gpu_params, float *gpu_m, float *gpu_v, const float *gpu_g
 /* BF16 copy (round to nearest even) of n FP32 values. */
 int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t stream);
 
+/* Aberration kernels in the CLT domain and their CltExtra records, from
+ * sampled point-spread functions.  The reference reads both from files its
+ * Java host generated (src/dtt8x8.cu:228-238, :364-378, formats
+ * [KV][KH][3][4][8][8] and CltExtra[KV][KH][3]); this builds them in those
+ * layouts, ready for tp_convert_correct_tiles.
+ *   gpu_psfs  : [cam] -> float[kernels_vert][kernels_hor][3][psf_height]
+ *               [psf_width]: the spatial kernel of each cell and color,
+ *               element [i][j] at (j - ox, i - oy) px from the origin
+ *   psf_height, psf_width: 1..32
+ *   center    : 0: origin (ox, oy) = the central element ((w-1)/2, (h-1)/2),
+ *               data_x = data_y = 0 and zero slopes; 1: origin = the
+ *               centroid of the kernel (a kernel summing to 0 keeps the
+ *               central element), data_x / data_y = the centroid relative to
+ *               the central element, dxc_dx, dxc_dy, dyc_dx, dyc_dy = its
+ *               central differences over the kernel grid (16 px per cell,
+ *               one-sided at the grid edges)
+ *   gpu_kernels        : [cam] -> CC, SC, CS, SS quadrants of
+ *                        K(k, l) = sum p e^{i u_k x} e^{j u_l y},
+ *                        u_k = pi (2k+1) / 16, [l][k] (output)
+ *   gpu_kernel_offsets : [cam] -> tp_clt_extra records; center_x, center_y
+ *                        = 16 kx - 8, 16 ky - 8 (output)
+ * Sums in FP64, stored as FP32. */
+int tp_psf_to_clt(const tp_geometry *geom, float **gpu_psfs, int psf_height, int psf_width,
+                  int center, float **gpu_kernels, float **gpu_kernel_offsets, tp_stream_t stream);
+
 /* 2-D 8x8 transform of every 8x8 tile of a width x height image, in place
  * of the tile (row = vertical frequency, column = horizontal frequency).
  * Replaces GPU_DTT24_DRV (src/dtt8x8.cuh:103-161), the reference's test

@@ -51,6 +51,7 @@ PROTOTYPES = {
     "tp_nn_backward": [_P, _P, _P, _P, _I, _F, _F, _F, _P, _P, _P],
     "tp_nn_adam": [_P, _P, _P, _P, _I, _F, _F, _F, _F, _I, _P, _P],
     "tp_nn_pack_bf16": [_P, _I, _P, _P],
+    "tp_psf_to_clt": [_G, _P, _I, _I, _I, _P, _P, _P],
     "tp_dtt8x8": [_P, _P, _I, _I, _I, _I, _P],
     "tp_fill": [_P, _SZ, _F, _P],
 }

@@ -20,6 +20,7 @@
 #include "k_fused.hpp"
 #include "k_nn.hpp"
 #include "k_dtt.hpp"
+#include "k_psf.hpp"
 
 namespace {
 
@@ -818,6 +819,33 @@ int tp_nn_pack_bf16(const float *gpu_src, int n, uint16_t *gpu_dst, tp_stream_t 
     hipLaunchKernelGGL(tp::k_nn_pack, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), gpu_src, n,
                        gpu_dst);
     return check_launch("nn_pack_bf16");
+}
+
+int tp_psf_to_clt(const tp_geometry *geom, float **gpu_psfs, int psf_height, int psf_width,
+                  int center, float **gpu_kernels, float **gpu_kernel_offsets, tp_stream_t stream) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (!gpu_psfs || !gpu_kernels || !gpu_kernel_offsets)
+        return fail(TP_EINVAL, "tp_psf_to_clt: NULL pointer argument%s", "");
+    if (psf_height < 1 || psf_width < 1 || psf_height > tp::PSF_MAX || psf_width > tp::PSF_MAX)
+        return fail(TP_EINVAL, "tp_psf_to_clt: PSF sides must be 1..32%s", "");
+    if (center != 0 && center != 1) return fail(TP_EINVAL, "tp_psf_to_clt: center must be 0 or 1%s", "");
+    g_err.clear();
+    tp::PsfArgs a;
+    a.psfs = (const float *const *)gpu_psfs;
+    a.kernels = gpu_kernels;
+    a.koffs = gpu_kernel_offsets;
+    a.KV = geom->kernels_vert;
+    a.KH = geom->kernels_hor;
+    a.h = psf_height;
+    a.w = psf_width;
+    a.center = center;
+    const int units = TP_NUM_CAMS * a.KV * a.KH * TP_NUM_COLORS;
+    hipLaunchKernelGGL(tp::k_psf_to_clt, dim3((units + 3) / 4), dim3(256), 0, as_stream(stream), a);
+    if ((rc = check_launch("psf_to_clt"))) return rc;
+    hipLaunchKernelGGL(tp::k_psf_slopes, dim3((units + 255) / 256), dim3(256), 0, as_stream(stream),
+                       gpu_kernel_offsets, a.KV, a.KH);
+    return check_launch("psf_slopes");
 }
 
 int tp_dtt8x8(float *gpu_dst, const float *gpu_src, int src_stride, int dtt_mode, int width,

@@ -63,6 +63,32 @@ def psf_kernels(sx, sy, theta):
     return k
 
 
+def psf_grid(geom, rng, sigma=(0.5, 2.0), size=15, offset=0.25):
+    """Sampled anisotropic Gaussian PSFs of every cell and color,
+    float32 [cams][KV][KH][3][size][size]: principal sigmas U[sigma], random
+    orientation, centre U[-offset, offset] px from the central element,
+    normalised to sum 1 (the input of tp_psf_to_clt)."""
+    shape = (NUM_CAMS, geom.kernels_vert, geom.kernels_hor, NUM_COLORS)
+    sx = rng.uniform(sigma[0], sigma[1], shape)
+    sy = rng.uniform(sigma[0], sigma[1], shape)
+    th = rng.uniform(0, np.pi, shape)
+    cx = rng.uniform(-offset, offset, shape)
+    cy = rng.uniform(-offset, offset, shape)
+    r = (size - 1) / 2.0
+    g = np.arange(size, dtype=np.float64) - r
+    out = np.empty(shape + (size, size), np.float32)
+    for cam in range(NUM_CAMS):  # one camera at a time keeps the temporaries small
+        e = lambda a: a[cam][..., None, None]  # noqa: E731
+        xs = g[None, None, None, None, :] - e(cx)
+        ys = g[None, None, None, :, None] - e(cy)
+        c, s_ = np.cos(e(th)), np.sin(e(th))
+        a = c * xs + s_ * ys
+        b = -s_ * xs + c * ys
+        p = np.exp(-0.5 * ((a / e(sx)) ** 2 + (b / e(sy)) ** 2))
+        out[cam] = p / p.sum(axis=(-2, -1), keepdims=True)
+    return out
+
+
 def identity_kernels(geom):
     """K_cc = 1, others 0: the aberration step becomes the identity."""
     k = np.zeros((NUM_CAMS, geom.kernels_vert, geom.kernels_hor, NUM_COLORS, 4, 8, 8), np.float32)

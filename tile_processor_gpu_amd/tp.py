@@ -346,6 +346,17 @@ class FrameBuffers:
             self.corr_idx[:len(ci)].copy_(torch.from_numpy(ci))
             self.num_corr = len(ci)
 
+    def kernels_from_psfs(self, psfs, center=True, stream=None):
+        """Fill self.kernels and self.koffs from sampled PSFs, a device tensor
+        [cams][KV][KH][3][h][w] (tp_psf_to_clt)."""
+        if psfs.dim() != 6 or psfs.shape[0] != NUM_CAMS:
+            raise ValueError("psfs must be [cams][KV][KH][3][h][w]")
+        psfs = psfs.contiguous()
+        p_psfs = ptr_array([psfs[c] for c in range(NUM_CAMS)])
+        psf_to_clt(self.geom, p_psfs, psfs.shape[4], psfs.shape[5], int(center), self.p_kernels,
+                   self.p_koffs, stream)
+        return p_psfs  # keep alive until the work is done
+
     def convert(self, lpf_mask=0, stream=None):
         convert_correct_tiles(self.geom, self.p_koffs, self.p_kernels, self.p_images, self.tasks,
                               self.p_clt, self.dstride, self.num_tiles, lpf_mask, stream)
@@ -453,3 +464,15 @@ def dtt8x8(gpu_dst, gpu_src, src_stride, dtt_mode, width, height, stream=None):
     _check_dev(gpu_dst, gpu_src)
     _lib.call("tp_dtt8x8", _p(gpu_dst), _p(gpu_src), int(src_stride), int(dtt_mode), int(width),
               int(height), _stream(stream))
+
+
+def psf_to_clt(geom, gpu_psfs, psf_height, psf_width, center, gpu_kernels, gpu_kernel_offsets,
+               stream=None):
+    """CLT-domain kernels [cam][KV][KH][3][4][8][8] and CltExtra records of
+    sampled PSFs [cam][KV][KH][3][h][w] (pointer arrays), the inputs of
+    convert_correct_tiles (the reference's .kernel / .kernel_offsets files,
+    src/dtt8x8.cu:228-238)."""
+    _check_dev(gpu_psfs, gpu_kernels, gpu_kernel_offsets)
+    g = geom.c()
+    _lib.call("tp_psf_to_clt", ctypes.byref(g), _p(gpu_psfs), int(psf_height), int(psf_width),
+              int(center), _p(gpu_kernels), _p(gpu_kernel_offsets), _stream(stream))
