@@ -55,11 +55,13 @@ def log(*a):
 
 
 def hbm_ceilings(reps=10, mib=1024):
-    """Practical HBM rates (GB/s, median of reps) of plain streaming kernels
-    (libtp_probe.so): pure read, pure write, copy, and the forward CLT's mix
-    of one byte read per three written.  No access pattern reaches the
-    8 TB/s peak; these say how far the measured kernels are from what the
-    memory system delivers."""
+    """Practical HBM rates (GB/s, median of reps) of one-shot streaming
+    kernels (libtp_probe.so, one 4 KB chunk per workgroup over a grid that
+    covers the buffer, the fastest form found): pure read, pure write, copy,
+    and the forward CLT's mix of one byte read per three written, each with
+    plain and non-temporal stores (the better one counts).  No access
+    pattern reaches the 8 TB/s peak; these say how far the measured kernels
+    are from what the memory system delivers."""
     import ctypes
     import torch
     from tile_processor_gpu_amd import build as tpb
@@ -75,11 +77,11 @@ def hbm_ceilings(reps=10, mib=1024):
     out = {}
     for name, kind, moved in (("read", 0, 1), ("write", 1, 1), ("copy", 2, 2), ("mix_1r3w", 3, 4)):
         best = None
-        for bpc in (4, 8):
+        for nt in ((0,) if kind == 0 else (0, 1)):
             def run():
                 rc = lib.tp_probe_hbm(kind, ctypes.c_void_p(src.data_ptr()),
                                       ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(sink.data_ptr()),
-                                      nbytes, bpc, st)
+                                      nbytes, nt, st)
                 if rc != 0:
                     raise RuntimeError("tp_probe_hbm failed")
             ms = median_ms(run, reps, 2)
