@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_corr_combine(CombineArgs a) {
             }
         }
     };
-    if (TP_COMBINE_PIPE && a.stride == (size_t)nn) {
+    if (a.stride == (size_t)nn) {
         // Contiguous rows, software-pipelined: the first chunk of the
         // wavefront's next group is requested before the current group is
         // summed.  Later chunks of a group (more than CHUNK entries, only

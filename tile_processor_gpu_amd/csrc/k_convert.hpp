@@ -5,12 +5,10 @@
 //
 // Work decomposition (gfx950, wave64): a "unit" is one (tile, camera, color)
 // 16x16 -> 4 x 8x8 transform.  8 lanes own a unit (lane = one row or column
-// of the 8x8 folded block), so a wavefront runs 8 units side by side.  A
-// workgroup is 3 wavefronts, one per Bayer color, over 2 tiles x 4 cameras;
-// it loops over consecutive tile pairs of a contiguous chunk.  Tiles are
-// visited by column pairs within strips of TP_STRIP_ROWS rows (for a raster
-// task list), so the tiles that share one 16x16-px aberration kernel cell
-// meet in one workgroup or in workgroups that run at the same time.
+// of the 8x8 folded block), so a wavefront runs 8 units side by side.  The
+// staged kernel k_convert_cam gives a workgroup 8 neighbouring tiles of one
+// tile row and its 4 wavefronts the 4 cameras of them; a wavefront converts
+// R, B, then G of its 8 tiles.
 //
 // Per unit:
 //   1. lane = folded column k gathers its 8 (R, B) or 16 (G) Bayer samples,
@@ -22,7 +20,7 @@
 //      (register reversal + one DPP mirror across the unit's lanes);
 //   6. bicomplex multiply by the aberration kernel (2 x float4 loads per
 //      quadrant), horizontal then vertical phase rotation, optional LPF;
-//   7. 2 x float4 stores per quadrant.
+//   7. the 1 KB block goes out through LDS as whole 128-byte lines.
 #pragma once
 
 #include "tp_common.hpp"
@@ -40,7 +38,6 @@ struct ConvertArgs {
     size_t dstride;
     int num_tiles;
     int lpf_mask;
-    int pairs_per_block;
     // this lane's camera: entries of the per-camera pointer arrays, read
     // once per kernel (or work item) by resolve_cam rather than per unit
     const float *l_koffs, *l_kern, *l_img;
@@ -66,8 +63,8 @@ constexpr LpfTab make_lpf_tab() {
 // low-pass tables: lpf_data[R,B,G,mono], lpf_rb_corr, lpf_corr
 __constant__ LpfTab c_lpf = make_lpf_tab();
 
-// Processing order -> task index.  The grid is cut into strips of
-// TP_STRIP_ROWS tile rows; within a strip, tiles are visited by column pairs
+// Processing order -> task index (fused kernel).  The grid is cut into
+// strips of TP_STRIP_ROWS tile rows; within a strip, tiles are visited by column pairs
 // (rows of a pair top to bottom), so that tiles sharing a 16x16-px
 // aberration kernel cell meet in one workgroup or in workgroups that run at
 // the same time.  Identity past the last complete strip; a last odd column
@@ -88,53 +85,6 @@ __device__ __forceinline__ int swizzle_strip(int s, int TX, int num_tiles) {
         c = TX - 1;
     }
     return st * strip + r * TX + c;
-}
-
-// Group order (staged forward CLT): strips of SH tile rows, each cut into
-// groups of CW columns, a group visited row by row.  With CW = 4 and
-// 8 tiles per workgroup, a workgroup converts 2 rows x 4 columns: whole
-// 48 KB runs of the CLT rows per camera and color plane, as in the
-// column-pair order, while the SH / 2 workgroups of a group (which run at
-// the same time on one XCD) share the aberration kernel rows between them.
-// Identity past the last complete strip.  A bijection on [0, num_tiles).
-__device__ __forceinline__ int swizzle_group(int s, int TX, int num_tiles, int SH, int CW) {
-    const int strip = SH * TX;
-    if (s >= (num_tiles / strip) * strip) return s;
-    const int st = s / strip, q = s - st * strip;
-    const int g = q / (SH * CW);
-    const int c0 = g * CW, w = min(CW, TX - c0);
-    const int r = q - g * SH * CW;
-    return st * strip + (r / w) * TX + c0 + r % w;
-}
-
-// Panel order for a full raster task list (num_tiles == TX * TY): panels of
-// PW tile columns, each visited as swizzle_strip visits the whole grid
-// (strips of TP_STRIP_ROWS rows, column pairs top to bottom), so that the
-// kernel cells a strip shares with the next one are still in L2 (a panel's
-// row of cells is (PW / 2 + 1) x 12 KB).  A bijection on [0, TX * TY).
-// Arguments are wavefront-uniform, so it runs on the scalar unit.
-__device__ __forceinline__ int swizzle_panel(int s, int TX, int TY, int PW) {
-    constexpr int SH = TP_STRIP_ROWS;
-    const int nfull = TX / PW, per = PW * TY;
-    int p = s / per, w = PW;
-    if (p >= nfull) {
-        p = nfull;
-        w = TX - nfull * PW;
-    }
-    const int q = s - p * per;
-    const int st = q / (SH * w), r = q - st * SH * w;
-    const int h = min(SH, TY - st * SH);
-    const int full = (w >> 1) * 2 * h;
-    int row, x;
-    if (r < full) {
-        const int cp = r / (2 * h), sub = r - cp * 2 * h;
-        row = sub >> 1;
-        x = 2 * cp + (sub & 1);
-    } else {
-        row = r - full;
-        x = w - 1;
-    }
-    return (st * SH + row) * TX + p * PW + x;
 }
 
 // Per-lane constants of the horizontal fold: folded column li is fed by
@@ -309,15 +259,7 @@ __device__ __forceinline__ void unit_fetch(const ConvertArgs &a, const LaneFold 
     // in the row; windows reaching past the right/bottom edge read the
     // nearest pixel of the same Bayer color (itlx, itly >= 0 always).
     gfloat *img = gptr(a.l_img);
-    // pixel loads; with TP_IMG_NT they are non-temporal (evict-first in L2),
-    // leaving the aberration kernels, which 4 tiles share, resident longer
-    auto pix = [&](int o) -> float {
-#if TP_IMG_NT
-        return __builtin_nontemporal_load(img + o);
-#else
-        return img[o];
-#endif
-    };
+    auto pix = [&](int o) -> float { return img[o]; };
     const int ds = (int)a.dstride;
     const bool inside = (itlx + 15 < g.W) && (itly + 15 < g.H);
     auto rowoff = [&](int r) -> int {
@@ -368,18 +310,6 @@ __device__ __forceinline__ void unit_compute(const ConvertArgs &a, const LaneFol
     const int itlx = F.itlx, itly = F.itly;
     const float sh = F.sh, sv = F.sv;
     const float *kern = a.l_kern + (size_t)F.kfi * 256;  // read through gptr4
-#if TP_KERN_EARLY
-    // this lane's kernel row (4 quadrants x 8 bins), loaded before the
-    // window and DCT stages so that its latency overlaps them
-    f32x4 kpre[8];
-    {
-        gf32x4 *kp = gptr4(kern + li * 8);
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
-    }
-#endif
     float ahs, ahc, avs, avc;
     sincospif(sh * 0.0625f, &ahs, &ahc);
     sincospif(sv * 0.0625f, &avs, &avc);
@@ -500,46 +430,10 @@ __device__ __forceinline__ void unit_compute(const ConvertArgs &a, const LaneFol
     // (9) aberration kernel (bicomplex product), convolveTiles :2778-2827;
     // one half row (4 bins) of the 4 kernel quadrants at a time
     {
-#if TP_KERN_SHARE && !TP_KERN_EARLY
-        // Units u and u + 4 of a wavefront (lanes l and l + 32) are one
-        // camera and color of two neighbouring tiles.  When their kernel
-        // cells coincide (about half the time) only the lower unit loads the
-        // rows, and the upper one takes them across the wavefront halves
-        // (v_permlane32_swap: the upper half of the first result holds the
-        // lower half of the source).
-        f32x4 kpre[8];
-        {
-            const int kfi_lo = (int)__builtin_amdgcn_permlane32_swap((unsigned)F.kfi, (unsigned)F.kfi,
-                                                                      false, false)[0];
-            const bool share = __lane_id() >= 32 && kfi_lo == F.kfi;
-            gf32x4 *kp = gptr4(kern + li * 8);
-#pragma unroll
-            for (int i = 0; i < 8; i++) kpre[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-            if (!share) {
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-#pragma unroll
-                    for (int qq = 0; qq < 4; qq++) kpre[h * 4 + qq] = kp[qq * 16 + h];
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const unsigned v = __float_as_uint(kpre[i][c]);
-                    const float lo = __uint_as_float(__builtin_amdgcn_permlane32_swap(v, v, false, false)[0]);
-                    kpre[i][c] = share ? lo : kpre[i][c];
-                }
-        }
-#elif !TP_KERN_EARLY
         gf32x4 *kp = gptr4(kern + li * 8);
-#endif
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-#if TP_KERN_EARLY || TP_KERN_SHARE
-            const f32x4 K0 = kpre[h * 4], K1 = kpre[h * 4 + 1], K2 = kpre[h * 4 + 2], K3 = kpre[h * 4 + 3];
-#else
             const f32x4 K0 = kp[h], K1 = kp[16 + h], K2 = kp[32 + h], K3 = kp[48 + h];
-#endif
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
                 const int k = 4 * h + kk;
@@ -624,12 +518,9 @@ __device__ __forceinline__ void convert_unit(const ConvertArgs &a, const LaneFol
     unit_compute<COLOR, DCT>(a, lf, lds_c, lds_s, li, F, q0, q1, q2, q3, mt, ustride);
 }
 
-// LDS slice of one unit in the staged kernels: its 8 x 8 transposition
-// staging, then its 1 KB output block.  TP_SLICE_PAD floats of padding
-// would put the 8 units of a wavefront on different LDS banks (at a
-// 256-float stride the column reads of the transposition hit the same banks
-// from all units); measured without effect, so off.
-constexpr int kSlice = 256 + TP_SLICE_PAD;
+// LDS slice of one unit in the staged kernel: its 8 x 8 transposition
+// staging, then its 1 KB output block.
+constexpr int kSlice = 256;
 
 // Tile index of a task, or -1 for a task outside the tile grid (skipped).
 __device__ __forceinline__ int task_tile(const Geo &g, int txy) {
@@ -695,134 +586,11 @@ __device__ __forceinline__ void store_block(const float *slice, float *dst, int 
     wave_lds_sync();  // the slice is the next unit's staging
 }
 
-// Task of unit slot `unit` in tile pair p of the visiting order (clamped
-// to the last task past the end; `valid` says whether it is a real one).
-__device__ __forceinline__ int pair_task(const ConvertArgs &a, int p, int unit, bool &valid) {
-    const int s = p * 2 + (unit >> 2);
-    valid = s < a.num_tiles;
-    int task_num;
-    if (TP_PANEL_W > 0 && a.num_tiles == a.g.TX * a.g.TY) {
-        // the pair's two tiles from uniform values, then per lane
-        const int sp = __builtin_amdgcn_readfirstlane(p * 2);
-        const int t0 = swizzle_panel(sp, a.g.TX, a.g.TY, TP_PANEL_W);
-        const int t1 = sp + 1 < a.num_tiles ? swizzle_panel(sp + 1, a.g.TX, a.g.TY, TP_PANEL_W) : t0;
-        task_num = (unit >> 2) ? t1 : t0;
-        if (!valid) task_num = a.num_tiles - 1;
-    } else {
-        task_num = !valid ? a.num_tiles - 1
-                 : TP_CONVERT_GROUP_W > 0
-                     ? swizzle_group(s, a.g.TX, a.num_tiles, TP_CONVERT_GROUP_H, TP_CONVERT_GROUP_W)
-                     : swizzle_strip(s, a.g.TX, a.num_tiles);
-    }
-    return task_num;
-}
-
-// The pairs [p0, p1) of one wavefront (one color), software-pipelined:
-// while pair p is converted, the CltExtra of pair p + 1 (whose centre
-// arrived during pair p - 1) and the centre of pair p + 2 are in flight, and
-// (TP_CONVERT_EARLY_FETCH) the Bayer samples of pair p + 1 are requested
-// before pair p's stores, so that waiting for them does not wait for the
-// stores.
-template <int COLOR>
-__device__ __forceinline__ void convert_loop(const ConvertArgs &a, const LaneFold &lf, float *slice,
-                                             int li, int cam, int unit, int p0, int p1) {
-    // The loads are unconditional (past the end they read the last task,
-    // unused) so that the wait counts are the same on every path and a
-    // wait for this pair's data does not also wait for the prefetches.
-    if (p0 >= p1) return;
-    bool v0, v1;
-    int t0 = pair_task(a, p0, unit, v0);
-    int t1 = pair_task(a, p0 + 1, unit, v1);
-    v1 = v1 && p0 + 1 < p1;
-    UnitPre u0, u1;
-    unit_centre(a, cam, t0, u0);
-    unit_centre(a, cam, t1, u1);
-    unit_extra<COLOR>(a, cam, u0);
-#if TP_CONVERT_EARLY_FETCH
-    UnitFetch f0;
-    unit_fetch<COLOR>(a, lf, u0, f0);
-#endif
-    for (int p = p0; p < p1; p++) {
-        bool v2;
-        const int t2 = pair_task(a, p + 2, unit, v2);
-        v2 = v2 && p + 2 < p1;
-        UnitPre u2;
-        unit_centre(a, cam, t2, u2);
-        unit_extra<COLOR>(a, cam, u1);
-        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of this pair's work
-        float q0[8], q1[8], q2[8], q3[8];
-#if TP_CONVERT_EARLY_FETCH
-        unit_compute<COLOR>(a, lf, slice, slice + 72, li, f0, q0, q1, q2, q3, {}, kSlice);
-#else
-        convert_unit<COLOR>(a, lf, slice, slice + 72, li, cam, t0, q0, q1, q2, q3, {}, kSlice, &u0);
-#endif
-        bool st;
-        float *dst = unit_dst<COLOR>(a, u0.txy, v0 && u0.task != 0, st);
-#if TP_CONVERT_LDS_STORE
-        stage_block(slice, li, q0, q1, q2, q3);
-#endif
-#if TP_CONVERT_EARLY_FETCH
-        unit_fetch<COLOR>(a, lf, u1, f0);  // pair p + 1
-        __builtin_amdgcn_sched_barrier(0);  // its loads ahead of this pair's stores
-#endif
-#if TP_CONVERT_LDS_STORE
-        store_block(slice, dst, li, st);
-#else
-        if (st) store_clt_rows(dst, li, q0, q1, q2, q3);
-#endif
-        u0 = u1;
-        t0 = t1;
-        v0 = v1;
-        u1 = u2;
-        t1 = t2;
-        v1 = v2;
-    }
-}
-
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
-void k_convert_correct_tiles(ConvertArgs a) {
-    __shared__ __align__(16) float lds[3][8][kSlice];  // [wave][unit]: staging, then output block
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int unit = lane >> 3;
-    const int li = lane & 7;
-    const int cam = unit & 3;
-    const int npairs = (a.num_tiles + 1) >> 1;
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int p0 = blk * a.pairs_per_block;
-    const int p1 = min(npairs, p0 + a.pairs_per_block);
-    float *slice = &lds[wave][unit][0];
-    const LaneFold lf = lane_fold(li);
-    ConvertArgs la = a;
-    resolve_cam(la, cam);
-    if (wave == 0)
-        convert_loop<0>(la, lf, slice, li, cam, unit, p0, p1);
-    else if (wave == 1)
-        convert_loop<1>(la, lf, slice, li, cam, unit, p0, p1);
-    else
-        convert_loop<2>(la, lf, slice, li, cam, unit, p0, p1);
-}
-
-// All three colors of a unit in one wavefront (TP_CONVERT_RGB): the wavefront
-// converts R, B, then G of its 8 (tile, camera) units, so every wavefront
-// does the same work (G costs about twice R or B) and the pixel lines loaded
-// for R are still in L1 for B and G; the centre is loaded once per pair.
-template <int COLOR>
-__device__ __forceinline__ void rgb_unit(const ConvertArgs &a, const LaneFold &lf, float *slice, int li,
-                                         const UnitPre &u, bool valid) {
-    UnitFetch F;
-    unit_fetch<COLOR>(a, lf, u, F);
-    __builtin_amdgcn_sched_barrier(0);
-    float q0[8], q1[8], q2[8], q3[8];
-    unit_compute<COLOR>(a, lf, slice, slice + 72, li, F, q0, q1, q2, q3, {}, kSlice);
-    bool st;
-    float *dst = unit_dst<COLOR>(a, u.txy, valid, st);
-    stage_block(slice, li, q0, q1, q2, q3);
-    store_block(slice, dst, li, st);
-}
-
 // R, B and G of the wavefront's 8 units (u0: their centres, task words
-// loaded; v0: whether the units' slots are real list entries).
+// loaded; v0: whether the units' slots are real list entries).  Each
+// color's samples are requested before the previous color's stores: loads
+// and stores share one completion counter, so a wait for samples issued
+// after the stores would also wait for the stores.
 __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold &lf, float *slice, int li,
                                           int cam, const UnitPre &u0, bool v0) {
     UnitPre ur = u0, ub = u0, ug = u0;
@@ -831,8 +599,6 @@ __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold 
     unit_extra<2>(la, cam, ug);
     __builtin_amdgcn_sched_barrier(0);
     const bool valid = v0 && ur.task != 0;
-#if TP_CONVERT_EARLY_FETCH
-    // each color's samples requested before the previous color's stores
     UnitFetch F;
     float q0[8], q1[8], q2[8], q3[8];
     bool st;
@@ -855,95 +621,31 @@ __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold 
     dst = unit_dst<2>(la, ug.txy, valid, st);
     stage_block(slice, li, q0, q1, q2, q3);
     store_block(slice, dst, li, st);
-#else
-    rgb_unit<0>(la, lf, slice, li, ur, valid);
-    rgb_unit<1>(la, lf, slice, li, ub, valid);
-    rgb_unit<2>(la, lf, slice, li, ug, valid);
-#endif
 }
 
+// Staged forward CLT: workgroup b converts tiles 8 b .. 8 b + 7 of the task
+// list (8 neighbouring tiles of a tile row for a raster list), wavefront w
+// camera w of them.  A wavefront's 8 units are 8 neighbouring tiles of one
+// camera, so units that share an aberration kernel cell (2 x 2 tiles per
+// cell) or image lines load the same addresses in the same instruction, and
+// the camera's pointers are wavefront-uniform.  Slots past the end of the
+// list convert the last task and store nothing.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
-void k_convert_rgb(ConvertArgs a) {
-    __shared__ __align__(16) float lds[4][8][kSlice];  // [wave][unit]: staging, then output block
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int unit = lane >> 3;
-    const int li = lane & 7;
-    const int cam = unit & 3;
-    const int npairs = (a.num_tiles + 1) >> 1;
-    const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
-    const int p0 = w * a.pairs_per_block;
-    const int p1 = min(npairs, p0 + a.pairs_per_block);
-    if (p0 >= p1) return;
-    float *slice = &lds[wave][unit][0];
-    const LaneFold lf = lane_fold(li);
-    ConvertArgs la = a;
-    resolve_cam(la, cam);
-    bool v0;
-    int t0 = pair_task(la, p0, unit, v0);
-    UnitPre u0;
-    unit_centre(la, cam, t0, u0);
-    for (int p = p0; p < p1; p++) {
-        bool v1;
-        const int t1 = pair_task(la, p + 1, unit, v1);
-        v1 = v1 && p + 1 < p1;
-        UnitPre u1;
-        unit_centre(la, cam, t1, u1);  // the next pair's centre in flight
-        rgb_units(la, lf, slice, li, cam, u0, v0);
-        u0 = u1;
-        t0 = t1;
-        v0 = v1;
-    }
-}
-
-
-// Camera per wavefront (TP_CONVERT_CAMWAVE): the 4 wavefronts of a
-// workgroup convert the 4 cameras of the same 8 tiles, a block of
-// TP_CAM_BLOCK_H x TP_CAM_BLOCK_W tiles in the group order.  A wavefront's 8
-// units are 8 neighbouring tiles of one camera, so units that share an
-// aberration kernel cell (2 x 2 tiles per cell) or image lines load the same
-// addresses in the same instruction, and the camera's pointers are
-// wavefront-uniform.
-__device__ __forceinline__ int block_task(const ConvertArgs &a, int b, int unit, bool &valid) {
-    const int s = b * 8 + unit;
-    valid = s < a.num_tiles;
-    return valid ? swizzle_group(s, a.g.TX, a.num_tiles, TP_CAM_BLOCK_H, TP_CAM_BLOCK_W) : a.num_tiles - 1;
-}
-
-__global__ __launch_bounds__(64 * TP_CAM_WG_WAVES) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_cam(ConvertArgs a) {
-    static_assert(TP_CAM_BLOCK_H * TP_CAM_BLOCK_W == 8, "8 tiles per block");
-    static_assert(TP_CAM_WG_WAVES == 4 || TP_CAM_WG_WAVES == 1, "wavefronts per workgroup");
-    __shared__ __align__(16) float lds[TP_CAM_WG_WAVES][8][kSlice];  // [wave][unit]: staging, then output block
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ __align__(16) float lds[4][8][kSlice];  // [wave][unit]: staging, then output block
+    const int cam = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
     const int li = lane & 7;
-    // with one wavefront per workgroup, workgroup 4 k + c is camera c of block group k
-    const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const int cam = TP_CAM_WG_WAVES == 4 ? wave : (wg & 3);
-    const int nblk = (a.num_tiles + 7) >> 3;
-    const int b0 = (TP_CAM_WG_WAVES == 4 ? wg : wg >> 2) * a.pairs_per_block;
-    const int b1 = min(nblk, b0 + a.pairs_per_block);
-    float *slice = &lds[wave][unit][0];
+    const int s = xcd_remap(blockIdx.x, gridDim.x) * 8 + unit;
+    const bool valid = s < a.num_tiles;
+    float *slice = &lds[cam][unit][0];
     const LaneFold lf = lane_fold(li);
     ConvertArgs la = a;
     resolve_cam(la, cam);
-    bool v0;
-    int t0 = block_task(la, b0, unit, v0);
     UnitPre u0;
-    unit_centre(la, cam, t0, u0);
-    for (int b = b0; b < b1; b++) {
-        bool v1;
-        const int t1 = block_task(la, b + 1, unit, v1);
-        v1 = v1 && b + 1 < b1;
-        UnitPre u1;
-        unit_centre(la, cam, t1, u1);  // the next block's centre in flight
-        rgb_units(la, lf, slice, li, cam, u0, v0);
-        u0 = u1;
-        t0 = t1;
-        v0 = v1;
-    }
+    unit_centre(la, cam, valid ? s : a.num_tiles - 1, u0);
+    rgb_units(la, lf, slice, li, cam, u0, valid);
 }
 
 }  // namespace tp

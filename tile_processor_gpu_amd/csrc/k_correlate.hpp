@@ -218,7 +218,6 @@ void k_correlate2D(CorrArgs a) {
         corr_finish(c0, c1, c2, c3, L, li, fz2);
         wave_lds_sync();
         const int vflag = valid ? 1 : 0;
-#if TP_CORR_BLOCK_STORE
         if (dense && __all(vflag)) {
             // 8 consecutive full 15 x 15 rows, contiguous in memory (7200 B,
             // 16-byte aligned): float4 q of the block from the staged tiles
@@ -235,7 +234,6 @@ void k_correlate2D(CorrArgs a) {
             wave_lds_sync();
             continue;
         }
-#endif
         for (int u = 0; u < 8; u++) {
             if (!__shfl(vflag, u * 8)) continue;
             const float *src = &lds[wave][u][0][0];

@@ -90,10 +90,7 @@ void k_fused(FusedArgs a) {
     constexpr int NSLOT = 6 * GT;
     // [tile][camera][color][4 x 8 x 8]; a unit's slice also serves as its
     // transposition staging (2 x 72 floats) before its CLT is written there
-    // each (tile, camera, color) block takes FB floats; TP_FUSED_PAD would
-    // put the 8 units of a wavefront on different LDS banks (measured
-    // without effect, so 0)
-    constexpr int FB = 256 + TP_FUSED_PAD;
+    constexpr int FB = 256;
     __shared__ __align__(16) float clt_lds[GT * 4 * 3 * FB];
     __shared__ int s_item, s_frame;
     const int wave = threadIdx.x >> 6;
@@ -177,23 +174,17 @@ void k_fused(FusedArgs a) {
         ca.dstride = a.dstride;
         ca.num_tiles = fr.num_tiles;
         ca.lpf_mask = a.lpf_mask;
-        ca.pairs_per_block = 0;
         resolve_cam(ca, cam);
         const int s0 = item * a.tiles_per_item;  // item: index within the frame
         const int s1 = min(fr.num_tiles, s0 + a.tiles_per_item);
 
         // This wavefront's jobs (color, tile pair) in every group of GT tiles:
         // see (1) below.
-        constexpr int NW = TP_FUSED_WG_WAVES;
-        constexpr bool BAL = TP_FUSED_BALANCE && GT == 4 && NW == 3;
-        constexpr bool W4 = NW == 4 && GT == 4;
-        const int njobs = W4 ? (wave < 2 ? 1 : 2) : (NW == 4 && wave == 3) ? 0 : GT / 2;
-        auto job_color = [&](int job) {
-            return W4 ? (wave < 2 ? 2 : wave - 2) : BAL ? (wave == 2 ? 1 : job == 0 ? 2 : 0) : wave;
-        };
-        auto job_pass = [&](int job) {
-            return W4 ? (wave < 2 ? wave : job) : BAL ? (wave == 2 ? job : wave) : job;
-        };
+        static_assert(TP_FUSED_WG_WAVES == 4, "4 wavefronts per workgroup");
+        constexpr bool W4 = GT == 4;
+        const int njobs = W4 ? (wave < 2 ? 1 : 2) : wave == 3 ? 0 : GT / 2;
+        auto job_color = [&](int job) { return W4 ? (wave < 2 ? 2 : wave - 2) : wave; };
+        auto job_pass = [&](int job) { return W4 ? (wave < 2 ? wave : job) : job; };
 
         for (int gs = s0; gs < s1; gs += GT) {
             // (1) forward CLT of the group into LDS
@@ -202,10 +193,8 @@ void k_fused(FusedArgs a) {
             // together.  G costs about twice R or B.  With 4 wavefronts and
             // GT = 4 (two tile pairs): wavefronts 0 and 1 do G of pair 0 /
             // pair 1, wavefront 2 does R and wavefront 3 B of both pairs.
-            // With 3 wavefronts, GT = 4 and TP_FUSED_BALANCE: wavefronts 0
-            // and 1 do G then R of pair 0 / pair 1, wavefront 2 does B of
-            // both pairs.  Otherwise wavefront w converts color w of every
-            // pair (with 4 wavefronts and GT = 2, wavefront 3 rests).
+            // With GT = 2, wavefront w converts color w of the pair and
+            // wavefront 3 rests.
 #pragma unroll 1
             for (int job = 0; job < njobs; job++) {
                 const int color = job_color(job);

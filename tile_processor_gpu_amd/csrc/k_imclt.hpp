@@ -44,17 +44,10 @@ template <int NQ = 4>
 __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], float (&q2)[8],
                                                 float (&q3)[8], float *L, int li) {
     static_assert(NQ == 1 || NQ == 2 || NQ == 4, "quadrants per transposition");
-#if TP_IMCLT_FAST
     dct4_8(q0);
     dst4_8(q1);
     dct4_8(q2);
     dst4_8(q3);
-#else
-    dtt8<0>(q0);
-    dtt8<1>(q1);
-    dtt8<0>(q2);
-    dtt8<1>(q3);
-#endif
     float *q[4] = {q0, q1, q2, q3};
 #pragma unroll
     for (int q0i = 0; q0i < 4; q0i += NQ) {
@@ -71,17 +64,10 @@ __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], 
             for (int i = 0; i < 8; i++) q[q0i + j][i] = L[j * 72 + i * 8 + li];
         wave_lds_sync();
     }
-#if TP_IMCLT_FAST
     dct4_8(q0);
     dct4_8(q1);
     dst4_8(q2);
     dst4_8(q3);
-#else
-    dtt8<0>(q0);
-    dtt8<0>(q1);
-    dtt8<1>(q2);
-    dtt8<1>(q3);
-#endif
 }
 
 // Step 5: calls emit(row, col, weight, sum) for the 32 outputs of lane li
@@ -163,12 +149,12 @@ __global__ __launch_bounds__(64) void k_imclt_pass(Geo g, const float *clt, floa
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_IMCLT_WAVES)))
 void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
                                                      size_t dstride) {
-    // the block's sums (pitch AP), then 8 rows of discard space: with
-    // TP_IMCLT_REDIRECT a lane's outputs outside the block (whole 8-row halves
-    // of a column) are sent there, which costs no per-output test
+    // the block's sums (pitch AP), then 8 rows of discard space: a lane's
+    // outputs outside the block (whole 8-row halves of a column) are sent
+    // there, which costs no per-output test
     constexpr int AP = 65, JUNK = 64 * AP;
     __shared__ __align__(16) float acc[JUNK + 8 * AP];
-    __shared__ __align__(16) float lds[4][8][TP_IMCLT_NQ][72];
+    __shared__ __align__(16) float lds[4][8][1][72];  // one quadrant per transposition
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int unit = lane >> 3, li = lane & 7;
     const int PW = g.W + 8, PH = g.H + 8;
@@ -199,31 +185,24 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
             float q0[8], q1[8], q2[8], q3[8];
             const int tyc = valid ? ty : 0, txc = valid ? tx : 0;
             load_rows(cl + ((size_t)(tyc * g.TX + txc) * TP_NUM_COLORS + color) * 256, li, q0, q1, q2, q3);
-            imclt_transform<TP_IMCLT_NQ>(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
+            imclt_transform<1>(q0, q1, q2, q3, &lds[wave][unit][0][0], li);
             if (valid) {
                 const int oy = ty * 8 - Y0, ox = tx * 8 - X0;  // -8 .. 56
-                if (TP_IMCLT_REDIRECT) {
-                    // the lane's 2 columns (as in imclt_unfold) x 2 row halves:
-                    // start of each in acc, or in the discard rows
-                    const int ca = li >= 4 ? li - 4 : 11 - li, cb = li >= 4 ? 11 - li : 12 + li;
-                    int base[2][2];
+                // the lane's 2 columns (as in imclt_unfold) x 2 row halves:
+                // start of each in acc, or in the discard rows
+                const int ca = li >= 4 ? li - 4 : 11 - li, cb = li >= 4 ? 11 - li : 12 + li;
+                int base[2][2];
 #pragma unroll
-                    for (int cs = 0; cs < 2; cs++)
+                for (int cs = 0; cs < 2; cs++)
 #pragma unroll
-                        for (int hh = 0; hh < 2; hh++) {
-                            const int c = cs ? cb : ca, x = ox + c, y0 = oy + 8 * hh;
-                            base[cs][hh] = ((unsigned)x < 64u && (unsigned)y0 < 64u) ? y0 * AP + x : JUNK + c;
-                        }
-                    imclt_unfold<false, true>(q0, q1, q2, q3, li, [&](int r, int, auto cs, float w, float d) {
-                        float *p = acc + base[decltype(cs)::value][r >> 3] + (r & 7) * AP;
-                        *p = fmaf(w, d, *p);
-                    });
-                } else {
-                    imclt_unfold(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) {
-                        const int y = oy + r, x = ox + c;
-                        if ((unsigned)y < 64u && (unsigned)x < 64u) acc[y * AP + x] = fmaf(w, d, acc[y * AP + x]);
-                    });
-                }
+                    for (int hh = 0; hh < 2; hh++) {
+                        const int c = cs ? cb : ca, x = ox + c, y0 = oy + 8 * hh;
+                        base[cs][hh] = ((unsigned)x < 64u && (unsigned)y0 < 64u) ? y0 * AP + x : JUNK + c;
+                    }
+                imclt_unfold<false, true>(q0, q1, q2, q3, li, [&](int r, int, auto cs, float w, float d) {
+                    float *p = acc + base[decltype(cs)::value][r >> 3] + (r & 7) * AP;
+                    *p = fmaf(w, d, *p);
+                });
             }
         }
         __syncthreads();

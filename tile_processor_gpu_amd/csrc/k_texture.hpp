@@ -84,7 +84,7 @@ struct TexShared {
     static constexpr int UNIT_WAVES = (TPW * 12 + 7) / 8;  // wavefronts of phase 1 (3 colors)
     float U[TPW * 12 * TEX_MS];  // [tile][cam * colors + color][256], un-windowed
     float cst[TC_N];
-    float scratch[UNIT_WAVES][8][TP_TEX_NQ][72];  // phase 1: imclt transposes
+    float scratch[UNIT_WAVES][8][1][72];  // phase 1: imclt transposes (one quadrant at a time)
 };
 
 // Per-pixel constants of thread t (pixel r = t / 16, c = t % 16), the same
@@ -332,7 +332,7 @@ __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, T
                 const float l = c_lpf.v[lw][li * 8 + kk];
                 q0[kk] *= l; q1[kk] *= l; q2[kk] *= l; q3[kk] *= l;
             }
-            imclt_transform<TP_TEX_NQ>(q0, q1, q2, q3, &sh.scratch[wave][lane >> 3][0][0], li);
+            imclt_transform<1>(q0, q1, q2, q3, &sh.scratch[wave][lane >> 3][0][0], li);
             float *m = sh.U + slot * TEX_MS;
             imclt_unfold<true>(q0, q1, q2, q3, li, [&](int r, int c, float w, float d) { m[r * 16 + c] = w * d; });
         }

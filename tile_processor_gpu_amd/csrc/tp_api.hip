@@ -244,22 +244,8 @@ int tp_convert_correct_tiles(const tp_geometry *geom, float **gpu_kernel_offsets
     a.dstride = dstride;
     a.num_tiles = num_tiles;
     a.lpf_mask = lpf_mask;
-    const int npairs = (num_tiles + 1) / 2;
-#if TP_CONVERT_CAMWAVE
-    a.pairs_per_block = TP_CONVERT_CAM_BPW;  // 8-tile blocks per workgroup (4 wavefronts, one per camera)
-    const int nblk = (num_tiles + 7) / 8;
-    const int grid = (nblk + a.pairs_per_block - 1) / a.pairs_per_block * (4 / TP_CAM_WG_WAVES);
-    (void)npairs;
-    hipLaunchKernelGGL(tp::k_convert_cam, dim3(grid), dim3(64 * TP_CAM_WG_WAVES), 0, as_stream(stream), a);
-#elif TP_CONVERT_RGB
-    a.pairs_per_block = TP_CONVERT_RGB_PPW;  // pairs per wavefront, 4 wavefronts per workgroup
-    const int grid = (npairs + 4 * a.pairs_per_block - 1) / (4 * a.pairs_per_block);
-    hipLaunchKernelGGL(tp::k_convert_rgb, dim3(grid), dim3(256), 0, as_stream(stream), a);
-#else
-    a.pairs_per_block = TP_CONVERT_PPB;
-    const int grid = (npairs + a.pairs_per_block - 1) / a.pairs_per_block;
-    hipLaunchKernelGGL(tp::k_convert_correct_tiles, dim3(grid), dim3(192), 0, as_stream(stream), a);
-#endif
+    // one workgroup (4 wavefronts, one per camera) per block of 8 tasks
+    hipLaunchKernelGGL(tp::k_convert_cam, dim3((num_tiles + 7) / 8), dim3(256), 0, as_stream(stream), a);
     return check_launch("convert_correct_tiles");
 }
 

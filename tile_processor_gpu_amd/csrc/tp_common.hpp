@@ -11,123 +11,22 @@
 
 #include "../../include/tp.h"
 
-// Tuning knobs (build-time): minimum waves per SIMD the register allocator
-// must allow, and work per workgroup.
-#ifndef TP_CONVERT_WAVES
-#define TP_CONVERT_WAVES 4
-#endif
-#ifndef TP_CORR_WAVES
-#define TP_CORR_WAVES 2
-#endif
-#ifndef TP_CONVERT_PPB
-#define TP_CONVERT_PPB 4   // tile pairs per workgroup
-#endif
-#ifndef TP_CORR_WG_WAVES
-#define TP_CORR_WG_WAVES 3  // correlation: wavefronts per workgroup (8 entries each per pass)
-#endif
-#ifndef TP_CORR_EPB
-#define TP_CORR_EPB (8 * TP_CORR_WG_WAVES)  // correlation entries per workgroup (multiple of 8 x waves)
-#endif
-#ifndef TP_FUSED_WAVES
-#define TP_FUSED_WAVES 1   // fused kernel: minimum waves per SIMD (1: compiler's choice)
-#endif
-#ifndef TP_FUSED_QWAVES
-#define TP_FUSED_QWAVES 3  // FP32 fused kernel: minimum waves per SIMD
-#endif
-#ifndef TP_FUSED_WG_WAVES
-#define TP_FUSED_WG_WAVES 4   // fused kernel: wavefronts per workgroup (3 or 4)
-#endif
-#ifndef TP_FUSED_BALANCE
-#define TP_FUSED_BALANCE 1 // fused kernel (4 tiles per group): color jobs spread evenly over wavefronts
-#endif
-#ifndef TP_KERN_EARLY
-#define TP_KERN_EARLY 0    // forward CLT: load the aberration kernel row before the pixels
-#endif
-#ifndef TP_CONVERT_RGB
-#define TP_CONVERT_RGB 1   // staged forward CLT: one wavefront converts all 3 colors of its units
-#endif
-#ifndef TP_CONVERT_RGB_PPW
-#define TP_CONVERT_RGB_PPW 2  // ... tile pairs per wavefront
-#endif
-#ifndef TP_CONVERT_CAMWAVE
-#define TP_CONVERT_CAMWAVE 1  // staged forward CLT: one camera per wavefront, 8 neighbouring tiles
-#endif
-#ifndef TP_CONVERT_CAM_BPW
-#define TP_CONVERT_CAM_BPW 1  // ... 8-tile blocks per workgroup
-#endif
-#ifndef TP_CAM_BLOCK_W
-#define TP_CAM_BLOCK_W 8      // ... block width in tiles (height 8 / width)
-#endif
-#define TP_CAM_BLOCK_H (8 / TP_CAM_BLOCK_W)
-#ifndef TP_CAM_WG_WAVES
-#define TP_CAM_WG_WAVES 4     // ... wavefronts per workgroup: 4 (the 4 cameras) or 1
-#endif
-#ifndef TP_SLICE_PAD
-#define TP_SLICE_PAD 0        // staged forward CLT: floats of padding per unit slice in LDS
-#endif
-#ifndef TP_FUSED_PAD
-#define TP_FUSED_PAD 0        // fused kernel: floats of padding per (tile, camera, color) block in LDS
-#endif
-#ifndef TP_IMCLT_NQ
-#define TP_IMCLT_NQ 1         // whole-frame inverse CLT: quadrants per LDS transposition (1, 2, 4)
-#endif
-#ifndef TP_IMCLT_WAVES
-#define TP_IMCLT_WAVES 5      // whole-frame inverse CLT: minimum waves per SIMD
-#endif
-#ifndef TP_IMCLT_REDIRECT
-#define TP_IMCLT_REDIRECT 1   // whole-frame inverse CLT: out-of-block outputs to discard rows (no per-output test)
-#endif
-#ifndef TP_IMCLT_FAST
-#define TP_IMCLT_FAST 1       // inverse CLT: DCT-IV / DST-IV through a 4-point complex DFT (0: matrix form)
-#endif
-#ifndef TP_TEX_WG_PER_CU
-#define TP_TEX_WG_PER_CU 5    // textures: workgroups per CU in the (list-looping) grid
-#endif
-#ifndef TP_TEX_NQ
-#define TP_TEX_NQ 1           // textures: quadrants per LDS transposition of the inverse CLT (1, 2, 4)
-#endif
-#ifndef TP_COMBINE_EPW
-#define TP_COMBINE_EPW 64     // pair combination: list entries per workgroup
-#endif
-#ifndef TP_COMBINE_PIPE
-#define TP_COMBINE_PIPE 1     // pair combination: next group's rows requested during the current one
-#endif
-#ifndef TP_TEX_TPW
-#define TP_TEX_TPW 2          // textures: list entries (tiles) per workgroup at a time (1, 2)
-#endif
-#ifndef TP_TEX_WAVES
-#define TP_TEX_WAVES 4        // textures: minimum waves per SIMD
-#endif
-#ifndef TP_KERN_SHARE
-#define TP_KERN_SHARE 0    // forward CLT: neighbouring tiles with one kernel cell load it once
-#endif
-#ifndef TP_STRIP_ROWS
-#define TP_STRIP_ROWS 2    // tile rows per strip of the forward-CLT visiting order
-#endif
-#ifndef TP_PANEL_W
-#define TP_PANEL_W 0       // forward CLT visiting order: panel width in tiles (0: full-width strips)
-#endif
-#ifndef TP_CONVERT_GROUP_W
-#define TP_CONVERT_GROUP_W 0  // staged forward CLT: group order width in tiles (0: column-pair strips)
-#endif
-#ifndef TP_CONVERT_GROUP_H
-#define TP_CONVERT_GROUP_H 4  // staged forward CLT: group order strip height in tile rows
-#endif
-#ifndef TP_IMG_NT
-#define TP_IMG_NT 0        // forward CLT: non-temporal pixel loads
-#endif
-#ifndef TP_CONVERT_LDS_STORE
-#define TP_CONVERT_LDS_STORE 1  // forward CLT output through LDS as whole 128-byte lines
-#endif
-#ifndef TP_CONVERT_EARLY_FETCH
-#define TP_CONVERT_EARLY_FETCH 1  // forward CLT: next unit's samples requested before this unit's stores
-#endif
-#ifndef TP_CORR_BLOCK_STORE
-#define TP_CORR_BLOCK_STORE 1  // correlation rows of radius 7 written as contiguous 8-row blocks
-#endif
-#ifndef TP_NT_STORE
-#define TP_NT_STORE 1      // write-once outputs bypass L2 retention (non-temporal)
-#endif
+// Launch shapes and occupancy targets, chosen by measurement on MI355X (the
+// alternatives tried and their timings are listed in the design notes; the
+// losing variants are not kept in the source).
+#define TP_CONVERT_WAVES 4     // forward CLT: minimum waves per SIMD
+#define TP_CORR_WAVES 2        // correlation: minimum waves per SIMD
+#define TP_CORR_WG_WAVES 3     // correlation: wavefronts per workgroup (8 entries each per pass)
+#define TP_CORR_EPB (8 * TP_CORR_WG_WAVES)  // correlation entries per workgroup
+#define TP_FUSED_WAVES 1       // fused kernel, matrix-core forms: minimum waves per SIMD (compiler's choice)
+#define TP_FUSED_QWAVES 3      // fused kernel, FP32 form: minimum waves per SIMD
+#define TP_FUSED_WG_WAVES 4    // fused kernel: wavefronts per workgroup
+#define TP_STRIP_ROWS 2        // fused kernel: tile rows per strip of the visiting order
+#define TP_IMCLT_WAVES 5       // whole-frame inverse CLT: minimum waves per SIMD
+#define TP_TEX_WG_PER_CU 5     // textures: workgroups per CU in the (list-looping) grid
+#define TP_TEX_TPW 2           // textures: list entries (tiles) per workgroup at a time
+#define TP_TEX_WAVES 4         // textures: minimum waves per SIMD
+#define TP_COMBINE_EPW 64      // pair combination: list entries per workgroup
 
 namespace tp {
 
@@ -287,25 +186,17 @@ __device__ __forceinline__ T gelem(const T *arr, int i) {
 }
 __device__ __forceinline__ gf32x4 *gptr4(const float *p) { return (gf32x4 *)p; }
 
-// 16-byte store of an output that is not read again by this kernel: with
-// TP_NT_STORE it is marked non-temporal, so it does not displace the inputs
-// that neighbouring workgroups re-read from L2.
+// 16-byte store of an output that is not read again by this kernel: marked
+// non-temporal, so it does not displace the inputs that neighbouring
+// workgroups re-read from L2.
 __device__ __forceinline__ void store4(float *p, float a, float b, float c, float d) {
     typedef __attribute__((address_space(1))) f32x4 gv4;
     const f32x4 v = {a, b, c, d};
-#if TP_NT_STORE
     __builtin_nontemporal_store(v, (gv4 *)p);
-#else
-    *(gv4 *)p = v;
-#endif
 }
 __device__ __forceinline__ void store1(float *p, float v) {
     typedef __attribute__((address_space(1))) float gv1;
-#if TP_NT_STORE
     __builtin_nontemporal_store(v, (gv1 *)p);
-#else
-    *(gv1 *)p = v;
-#endif
 }
 
 // Order LDS traffic between lanes of one wavefront.
@@ -319,21 +210,6 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ float mirror8(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141 /* row_half_mirror */,
                                                    0xf, 0xf, false));
-}
-
-// y = M x for an 8-point transform of the given kind (matrix form)
-template <int KIND>
-__device__ __forceinline__ void dtt8(float (&x)[8]) {
-    float y[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        float s = kTab.dtt[KIND][k][0] * x[0];
-#pragma unroll
-        for (int n = 1; n < 8; n++) s = fmaf(kTab.dtt[KIND][k][n], x[n], s);
-        y[k] = s;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) x[k] = y[k];
 }
 
 // 8-point DCT-IV, y[k] = 1/2 sum_n x[n] cos(pi/8 (n+1/2)(k+1/2)), through a
