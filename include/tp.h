@@ -97,6 +97,17 @@ typedef struct tp_geometry {
 
 typedef void *tp_stream_t; /* a hipStream_t; NULL = the null stream */
 
+/* Layouts the reference's buffers and files fix (tp_task 40 bytes, CltExtra
+ * 32 bytes), checked at compile time in C and C++. */
+#ifdef __cplusplus
+#define TP_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define TP_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+TP_STATIC_ASSERT(sizeof(tp_task) == 40, "tp_task must be 40 bytes (src/TileProcessor.cuh:179-186)");
+TP_STATIC_ASSERT(sizeof(tp_clt_extra) == 32, "tp_clt_extra must be 32 bytes (src/TileProcessor.cuh:187-196)");
+TP_STATIC_ASSERT(sizeof(tp_geometry) == 16, "tp_geometry: 4 int32");
+
 /* Text of the last error on this thread ("" if none). */
 const char *tp_last_error(void);
 
@@ -337,6 +348,7 @@ typedef struct tp_fused_config {
     int32_t blocks_per_cu;    /* stream form: resident workgroups per CU, 1..8 (default 4) */
     int32_t dct;              /* forward DCT-IV/DST-IV: 0 FP32 butterflies (default) */
 } tp_fused_config;
+TP_STATIC_ASSERT(sizeof(tp_fused_config) == 16, "tp_fused_config: 4 int32");
 
 /* One frame of the stream form; an array of these lives in device memory.
  * images / kernels / kernel_offsets / clt are device arrays of TP_NUM_CAMS
@@ -354,6 +366,7 @@ typedef struct tp_frame {
     int32_t num_tiles;
     int32_t num_items;
 } tp_frame;
+TP_STATIC_ASSERT(sizeof(tp_frame) == 64, "tp_frame: 7 pointers and 2 int32");
 
 /* Forward CLT + correlation of one frame.  Arguments as for
  * tp_convert_correct_tiles and tp_correlate2D; gpu_clt and gpu_corrs may be

@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# TP_ORACLE_LIB: another build of the same source (the sanitizer build)
+LIB_PATH = os.environ.get("TP_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _VP = ctypes.c_void_p
 

@@ -117,3 +117,17 @@ def test_stream_noalloc_rejects_too_many_frames():
     rc, msg = rc_err("tp_stream_convert_correlate_noalloc", *args(4097))
     assert rc == -1 and "4096" in msg
     assert rc_err("tp_stream_convert_correlate_noalloc", *args(0))[0] == -1
+
+
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    """include/tp.h is plain C (and C++): its struct-size assertions hold
+    for both compilers."""
+    import shutil
+    import subprocess
+    src = tmp_path / "t.c"
+    src.write_text('#include "tp.h"\nint main(void) { tp_task t; (void)t; return 0; }\n')
+    inc = os.path.join(ROOT, "include")
+    for cc, args in (("gcc", ["-std=c11"]), ("g++", ["-std=c++17", "-x", "c++"])):
+        if not shutil.which(cc):
+            pytest.skip(f"{cc} not found")
+        subprocess.check_call([cc, *args, "-Wall", "-Werror", "-I", inc, "-fsyntax-only", str(src)])
