@@ -26,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix-core peak
+NN_FLOP_PER_TILE = 2 * 11808 * 225  # disparity network: MAC per pixel x pixels x 2
 FAT_ZERO = 0.05
 SCALES = (0.25, 0.25, 0.5)
 RADIUS = 7
@@ -146,11 +148,22 @@ def err_vs_fp64(cfg, fb, idx, sample_tiles):
     return out
 
 
-def cpu_baseline(cfg, idx, rows, min_s=10.0, max_reps=200):
+def cpu_baseline(cfg, idx, rows, min_s=10.0, max_reps=200, threads=0):
     """FP32 oracle (OpenMP over tiles) on the first `rows` tile rows, run
     repeatedly for at least min_s seconds (at least 3 runs); the median run
-    is reported."""
+    is reported.  threads: OpenMP threads (0: all of the host's)."""
     from oracle import oracle as orc
+    g = cfg["geom"]
+    prev = orc.num_threads()
+    if threads:
+        orc.num_threads(threads)
+    try:
+        return _cpu_baseline(orc, cfg, idx, rows, min_s, max_reps)
+    finally:
+        orc.num_threads(prev)
+
+
+def _cpu_baseline(orc, cfg, idx, rows, min_s, max_reps):
     g = cfg["geom"]
     cores = orc.num_threads()
     n = rows * g.tilesx
@@ -391,8 +404,14 @@ def measure_stages(cfg, fb, idx, reps):
                       g.num_tiles, ctypes.c_void_p(fb.corrs.data_ptr()), fb.corr_stride,
                       ctypes.c_void_p(x.data_ptr()), tp._stream(None))
         out["nn_input_ms"] = round(median_ms(nn_input, max(3, reps // 4), warmup=1), 4)
-        out["nn_frame_ms_matrix_core_bf16"] = round(
-            median_ms(lambda: model.infer(x, g.num_tiles), max(3, reps // 4), warmup=1), 4)
+        ms_nn = median_ms(lambda: model.infer(x, g.num_tiles), max(3, reps // 4), warmup=1)
+        out["nn_frame_ms_matrix_core_bf16"] = round(ms_nn, 4)
+        # roofline of the inference kernel in FLOP terms: 11808 MAC per
+        # pixel (7 -> 32, 32 -> 32 and 32 -> 2 channels, 3 x 3) x 225 pixels
+        # per tile, against the 2.5 PFLOP/s dense BF16 matrix-core peak
+        flops = NN_FLOP_PER_TILE * g.num_tiles
+        out["nn_infer_TFLOPs"] = round(flops / (ms_nn * 1e-3) / 1e12, 1)
+        out["nn_infer_frac_of_bf16_peak"] = round(flops / (ms_nn * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
         out["nn_frame_ms_fp32"] = round(
             median_ms(lambda: model.forward(x, g.num_tiles), 3, warmup=1), 4)
         del x
@@ -442,11 +461,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the plain staged path (no HBM ceilings, fused or stream measurements)")
     ap.add_argument("--stream-frames", type=int, default=64)
-    ap.add_argument("--mode", choices=["frame", "copy", "fused", "texture"], default="frame",
+    ap.add_argument("--mode", choices=["frame", "copy", "fused", "texture", "nn"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes); fused: the fused kernel "
                          "alone; texture: generate_RBGA, then the whole-frame inverse CLT, each "
-                         "--steps times on the full frame (for profiling)")
+                         "--steps times on the full frame; nn: network inference on the full "
+                         "frame's tiles (for profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -473,6 +493,24 @@ def main():
         cfg, fb, idx, _ = setup_frame(args.seed)
         ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO), max(1, args.steps), args.warmup)
         print(json.dumps({"mode": "fused", "ms": round(ms, 4), "steps": args.steps}), flush=True)
+        return
+
+    if args.mode == "nn":  # network inference alone on the full frame's tiles, for profiling
+        import ctypes
+        from tile_processor_gpu_amd import _lib, nn
+        cfg, fb, idx, _ = setup_frame(args.seed)
+        g = cfg["geom"]
+        model = nn.DisparityCNN.load(nn.DEFAULT_WEIGHTS)
+        fb.fused(3, SCALES, FAT_ZERO)
+        x = torch.empty((g.num_tiles, 7, 225), device="cuda:0")
+        _lib.call("tp_nn_input", ctypes.byref(g.c()), ctypes.c_void_p(fb.tasks.data_ptr()), g.num_tiles,
+                  ctypes.c_void_p(fb.corrs.data_ptr()), fb.corr_stride, ctypes.c_void_p(x.data_ptr()),
+                  tp._stream(None))
+        ms = median_ms(lambda: model.infer(x, g.num_tiles), max(1, args.steps), args.warmup)
+        tf = NN_FLOP_PER_TILE * g.num_tiles / (ms * 1e-3) / 1e12
+        print(json.dumps({"mode": "nn", "ms": round(ms, 4), "TFLOPs": round(tf, 1),
+                          "frac_of_bf16_peak": round(tf / BF16_PEAK_TFLOPS, 4), "steps": args.steps}),
+              flush=True)
         return
 
     if args.mode == "texture":
@@ -574,10 +612,13 @@ def main():
         extra["pipeline_stages"] = measure_stages(cfg, fb, idx, max(5, K // 2))
         log(f"stages: {extra['pipeline_stages']}")
 
-    cpu = None
+    cpu = cpu1 = None
     if args.cpu_rows > 0:
         cpu = cpu_baseline(cfg, idx, args.cpu_rows)
         log(f"cpu baseline: {cpu}")
+        # one thread, on 8 tile rows (about 5 s)
+        cpu1 = cpu_baseline(cfg, idx, min(8, args.cpu_rows), min_s=5.0, threads=1)
+        log(f"cpu baseline, one thread: {cpu1}")
 
     value = K / total_s
     out = {
@@ -604,6 +645,7 @@ def main():
         "stages": stage,
         "err_vs_fp64": err,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "kernel_generation": kernel_gen,
     }
     out.update(extra)

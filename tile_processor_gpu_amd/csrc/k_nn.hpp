@@ -400,24 +400,35 @@ __global__ __launch_bounds__(256) void k_nn_backward(NNBackArgs a) {
 }
 
 // ----------------------------------------------------- inference (MFMA)
-// The same network with BF16 activations, the two 32-channel convolutions
-// as implicit GEMMs on v_mfma_f32_32x32x16_bf16 (D[co][pixel] = sum_k
-// W[co][k] X[k][pixel], k = (3x3 neighbour r, input channel)), FP32
-// accumulation; the 2-channel output convolution and the head in FP32 on
-// the vector units.
+// The same network with BF16 activations and all three convolutions on
+// the matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation):
+//   - the two 32-channel 3x3 convolutions as implicit GEMMs, D[co][pixel] =
+//     sum_k W[co][k] X[k][pixel], k = (3x3 neighbour r, input channel);
+//   - the 2-channel output convolution as a 1x1 product G[pixel][(co, r)] =
+//     sum_ci h2[pixel][ci] W3[co][ci][r] (18 of 32 columns used), then
+//     z[co](p) = b3[co] + sum_r G[p + off_r][(co, r)] (a 9-tap shift-add of
+//     G, zero outside the tile: h2 is zero there).
+// The head (softmax, disparity, confidence) runs on the vector units.
 //
-// Activations live in LDS channel-last on a zero-padded 17 x 17 grid
-// (pixel (y, x) at (y + 1) * 17 + x + 1), so the 8 consecutive channels a
-// lane contributes to one K slice are one 16-byte load at any neighbour.
-// Operand layouts (32x32x16): lane i holds A[i % 32][8 (i / 32) + e],
-// B[8 (i / 32) + e][i % 32] (e = 0..7) and D[8 (v / 4) + 4 (i / 32) + v % 4]
-// [i % 32] (v = 0..15).  Each wavefront takes 2 of a tile's 8 blocks of 32
-// pixels; the weights of both convolutions stay in registers.
+// Activations live in LDS channel-last, the inputs of the 3x3 convolutions
+// (x, h1) and G on a zero-padded 17 x 17 grid (pixel (y, x) at (y + 1) * 17
+// + x + 1), h2 unpadded (the 1x1 product reads only a wavefront's own
+// pixels), so the 8 consecutive channels a lane contributes to one K slice
+// are one 16-byte load.  Operand layouts (32x32x16): lane i holds
+// A[i % 32][8 (i / 32) + e], B[8 (i / 32) + e][i % 32] (e = 0..7) and
+// D[8 (v / 4) + 4 (i / 32) + v % 4][i % 32] (v = 0..15).  Each wavefront
+// takes 2 of a tile's 8 blocks of 32 pixels and runs their two MFMA chains
+// interleaved (independent accumulators); the weights stay in registers.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int NN_PAD = 17, NN_PADP = NN_PAD * NN_PAD;
+constexpr int NN_GS = 19;  // G row pitch in floats (18 used; odd: fewer bank conflicts)
+// pixel pitch of the 32-channel activations in BF16 elements: 80 bytes, so
+// the 16-byte reads of 8 consecutive pixels cover all 32 LDS banks (at 64
+// bytes they would share 8 banks, 4 lanes on each)
+constexpr int NN_CS = 40;
 
 __device__ __forceinline__ int nn_padded(int p) { return (p / 15 + 1) * NN_PAD + p % 15 + 1; }
 
@@ -438,22 +449,61 @@ __device__ __forceinline__ u32x4 nn_wfrag(const uint16_t *__restrict__ w, int co
     return v;
 }
 
-__device__ __forceinline__ uint32_t bf16pair(float a, float b) {
-    return (uint32_t)f2bf16(a) | ((uint32_t)f2bf16(b) << 16);
+// B fragment of the 1x1 output product, K step s: W3[co][ci][r] at
+// k = ci = 16 s + 8 (lane / 32) + e, column n = lane % 32 = co * 9 + r
+// (columns 18..31 zero)
+__device__ __forceinline__ u32x4 nn_w3frag(const uint16_t *__restrict__ prm, int lane, int s) {
+    const int n = lane & 31, co = n / 9, r = n - 9 * co;
+    uint32_t h[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int ci = 16 * s + 8 * (lane >> 5) + e;
+        h[e] = n < 18 ? (uint32_t)prm[NN_OFF_W3 + (co * NN_C + ci) * 9 + r] : 0u;
+    }
+    u32x4 v;
+#pragma unroll
+    for (int d = 0; d < 4; d++) v[d] = h[2 * d] | (h[2 * d + 1] << 16);
+    return v;
 }
 
-// one 32-pixel block of a 3x3 convolution: in = LDS activations with C
-// channels per pixel; NK chunks of 16 K values; A = weight fragments.
-// k ordering per chunk kc: C = 8: r = 2 kc + i / 32, channels 0..7;
-// C = 32: r = kc / 2, channels (kc % 2) 16 + 8 (i / 32) .. + 7.
+// (a, b) as two BF16 in one word, round to nearest even (one
+// v_cvt_pk_bf16_f32)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16pair(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// ReLU of two BF16 in one word: as 16-bit integers the negative values
+// (and -0) are the negative ones, so max(x, 0) per half (v_pk_max_i16)
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    const s16x2 z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), z));
+}
+
+__device__ __forceinline__ f32x16 nn_mfma(const u32x4 &a, const u32x4 &b, const f32x16 &c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Two 32-pixel blocks (pixels p0, p1 of this lane's column) of a 3x3
+// convolution: in = padded LDS activations with C channels per pixel (pitch
+// 8 for C = 8, NN_CS for C = 32); NK chunks of 16 K values; A = weight
+// fragments.  k ordering per chunk kc: C = 8: r = 2 kc + i / 32, channels
+// 0..7; C = 32: r = kc / 2, channels (kc % 2) 16 + 8 (i / 32) .. + 7.
+// Pixels past the tile read the zero ring.  The B fragments of chunk kc + 2
+// are requested while chunks kc and kc + 1 are multiplied (the LDS latency
+// is not covered by other wavefronts at 2 waves per SIMD).
 template <int C, int NK>
-__device__ __forceinline__ f32x16 nn_conv_block(const uint16_t *in, const u32x4 (&A)[NK], int p,
-                                               int lane, f32x16 acc) {
-    const bool valid = p < NN_P;
-    const int pp = valid ? nn_padded(p) : 0;  // ring pixel 0 is zero
+__device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 (&A)[NK], int p0, int p1,
+                                                int lane, f32x16 &acc0, f32x16 &acc1) {
+    constexpr int CS = C == 8 ? 8 : NN_CS;
+    constexpr int D = 2;  // prefetch distance in chunks
+    const int pp0 = p0 < NN_P ? nn_padded(p0) : -1, pp1 = p1 < NN_P ? nn_padded(p1) : -1;
     const int half = lane >> 5;
-#pragma unroll
-    for (int kc = 0; kc < NK; kc++) {
+    auto fetch = [&](int kc, u32x4 &b0, u32x4 &b1) {
         int r, c0;
         if (C == 8) {
             r = 2 * kc + half;
@@ -462,136 +512,195 @@ __device__ __forceinline__ f32x16 nn_conv_block(const uint16_t *in, const u32x4 
             r = kc >> 1;
             c0 = (kc & 1) * 16 + 8 * half;
         }
-        u32x4 b = {0u, 0u, 0u, 0u};
+        b0 = u32x4{0u, 0u, 0u, 0u};
+        b1 = u32x4{0u, 0u, 0u, 0u};
         if (r < 9) {
-            const int q = valid ? pp + (r / 3 - 1) * NN_PAD + (r % 3 - 1) : 0;
-            b = *(const u32x4 *)(in + q * C + c0);
+            const int d = (r / 3 - 1) * NN_PAD + (r % 3 - 1);
+            b0 = *(const u32x4 *)(in + (pp0 >= 0 ? pp0 + d : 0) * CS + c0);
+            b1 = *(const u32x4 *)(in + (pp1 >= 0 ? pp1 + d : 0) * CS + c0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, A[kc]),
-                                                      __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    };
+    u32x4 b0[D + 1], b1[D + 1];
+#pragma unroll
+    for (int kc = 0; kc < D && kc < NK; kc++) fetch(kc, b0[kc], b1[kc]);
+#pragma unroll
+    for (int kc = 0; kc < NK; kc++) {
+        if (kc + D < NK) fetch(kc + D, b0[(kc + D) % (D + 1)], b1[(kc + D) % (D + 1)]);
+        // the loads of chunk kc + D stay ahead of the products of chunk kc
+        __builtin_amdgcn_sched_barrier(0);
+        acc0 = nn_mfma(A[kc], b0[kc % (D + 1)], acc0);
+        acc1 = nn_mfma(A[kc], b1[kc % (D + 1)], acc1);
+        __builtin_amdgcn_sched_barrier(0);
     }
-    return acc;
 }
 
-// bias + ReLU, BF16, channel-last store of this lane's 16 channels of pixel p
-__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, int p, int lane) {
+// bias + ReLU, BF16, channel-last store of this lane's 16 channels of
+// pixel p at out[idx * NN_CS]; bias: the layer's 32 biases in LDS (this lane's
+// rows 8 j + 4 (lane / 32) .. + 3 are one float4 each)
+__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, const float *bias, int p,
+                                               int idx, int lane) {
     if (p >= NN_P) return;
-    uint16_t *o = out + nn_padded(p) * NN_C + 4 * (lane >> 5);
+    uint16_t *o = out + idx * NN_CS + 4 * (lane >> 5);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const uint32_t lo = bf16pair(fmaxf(acc[4 * j], 0.f), fmaxf(acc[4 * j + 1], 0.f));
-        const uint32_t hi = bf16pair(fmaxf(acc[4 * j + 2], 0.f), fmaxf(acc[4 * j + 3], 0.f));
+        const float4 b = *(const float4 *)(bias + 8 * j + 4 * (lane >> 5));
+        const uint32_t lo = relu_bf16x2(bf16pair(acc[4 * j] + b.x, acc[4 * j + 1] + b.y));
+        const uint32_t hi = relu_bf16x2(bf16pair(acc[4 * j + 2] + b.z, acc[4 * j + 3] + b.w));
         *(uint2 *)(o + 8 * j) = make_uint2(lo, hi);
     }
 }
 
-__global__ __launch_bounds__(256) void k_nn_infer(const uint16_t *__restrict__ prm,
-                                                  const float *__restrict__ x, int n, float *out) {
-    __shared__ __align__(16) uint16_t X0[NN_PADP * 8];
-    __shared__ __align__(16) uint16_t H1[NN_PADP * NN_C];
-    __shared__ __align__(16) uint16_t H2[NN_PADP * NN_C];
-    __shared__ __align__(16) float W3s[2 * 9 * NN_C];  // [co][r][ci]
+// All-lane sum / max of a wavefront: butterfly within each row of 16 lanes
+// by DPP (quad swaps 1 and 2, half-row mirror, row mirror: every lane of a
+// row ends with the row's result), then the 4 rows' results by readlane.
+template <int CTRL>
+__device__ __forceinline__ float nn_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float nn_row_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ float nn_wave_sum(float v) {
+    v += nn_dpp<0xB1>(v);   // quad_perm 1,0,3,2
+    v += nn_dpp<0x4E>(v);   // quad_perm 2,3,0,1
+    v += nn_dpp<0x141>(v);  // row_half_mirror
+    v += nn_dpp<0x140>(v);  // row_mirror
+    return (nn_row_lane(v, 0) + nn_row_lane(v, 16)) + (nn_row_lane(v, 32) + nn_row_lane(v, 48));
+}
+__device__ __forceinline__ float nn_wave_max(float v) {
+    v = fmaxf(v, nn_dpp<0xB1>(v));
+    v = fmaxf(v, nn_dpp<0x4E>(v));
+    v = fmaxf(v, nn_dpp<0x141>(v));
+    v = fmaxf(v, nn_dpp<0x140>(v));
+    return fmaxf(fmaxf(nn_row_lane(v, 0), nn_row_lane(v, 16)), fmaxf(nn_row_lane(v, 32), nn_row_lane(v, 48)));
+}
+
+// 2 workgroups per CU (70 KB of LDS each): 2 waves per SIMD, at most 256
+// registers per lane
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, int n, float *out) {
+    __shared__ __align__(16) uint16_t X0[NN_PADP * 8];   // x, padded, channel 7 zero
+    __shared__ __align__(16) uint16_t H1[NN_PADP * NN_CS];
+    __shared__ __align__(16) uint16_t H2[256 * NN_CS];   // unpadded, rows past the tile zero
+    __shared__ __align__(16) float Gs[NN_P * NN_GS];      // G, unpadded (pixel p at row p)
+    __shared__ __align__(16) float bias[2][NN_C];       // b1, b2
     __shared__ float red[16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int co = lane & 31, half = lane >> 5;
+    if (t < 2 * NN_C) bias[t >> 5][t & 31] = bf16f(prm[(t < NN_C ? NN_OFF_B1 : NN_OFF_B2 - NN_C) + t]);
     for (int i = t; i < NN_PADP * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
-    for (int i = t; i < NN_PADP * 16; i += 256) {
-        ((uint32_t *)H1)[i] = 0u;
-        ((uint32_t *)H2)[i] = 0u;
-    }
-    for (int i = t; i < 576; i += 256) {
-        const int c = i / 288, r = (i / 32) % 9, ci = i % 32;
-        W3s[i] = bf16f(prm[NN_OFF_W3 + (c * NN_C + ci) * 9 + r]);
-    }
-    u32x4 A1[5], A2[18];
+    for (int i = t; i < 256 * NN_CS / 2; i += 256) ((uint32_t *)H2)[i] = 0u;
+    for (int i = t; i < NN_PADP * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
+    u32x4 A1[5], A2[18], B3[2];
 #pragma unroll
     for (int kc = 0; kc < 5; kc++) A1[kc] = nn_wfrag<NN_IN>(prm + NN_OFF_W1, co, 0, 2 * kc + half, NN_IN);
 #pragma unroll
     for (int kc = 0; kc < 18; kc++)
         A2[kc] = nn_wfrag<NN_C>(prm + NN_OFF_W2, co, (kc & 1) * 16 + 8 * half, kc >> 1, NN_C);
-    f32x16 bias1, bias2;  // accumulator start values: bias of each row this lane holds
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int row = 8 * (v >> 2) + 4 * half + (v & 3);
-        bias1[v] = bf16f(prm[NN_OFF_B1 + row]);
-        bias2[v] = bf16f(prm[NN_OFF_B2 + row]);
-    }
+    B3[0] = nn_w3frag(prm, lane, 0);
+    B3[1] = nn_w3frag(prm, lane, 1);
     const float b30 = bf16f(prm[NN_OFF_B3]), b31 = bf16f(prm[NN_OFF_B3 + 1]);
     __syncthreads();
-    // thread t's input elements t + 256 k of a tile, requested one tile ahead
-    constexpr int XPT = (NN_IN * NN_P + 255) / 256;
-    float xn[XPT];
+    // thread t = pixel t: its 7 input channels of a tile, requested one tile
+    // ahead
+    const bool px = t < NN_P;
+    float xn[NN_IN];
     auto load_x = [&](int tile) {
-        const float *xt = x + (size_t)tile * NN_IN * NN_P;
+        const float *xt = x + (size_t)tile * NN_IN * NN_P + (px ? t : 0);
 #pragma unroll
-        for (int k = 0; k < XPT; k++) {
-            const int i = t + 256 * k;
-            xn[k] = (tile < n && i < NN_IN * NN_P) ? xt[i] : 0.f;
+        for (int c = 0; c < NN_IN; c++) xn[c] = (tile < n && px) ? xt[c * NN_P] : 0.f;
+    };
+    const f32x16 zero = {};
+    const int p0 = 64 * wave + co, p1 = p0 + 32;  // this lane's pixel column in the wave's 2 blocks
+    const int ppx = px ? nn_padded(t) : 0;
+    const unsigned nbm = px ? nbr_mask_yx(t / 15, t % 15) : 0u;
+    // the head of the previous tile: thread 0 combines the wavefronts'
+    // partial sums in red (after the first barrier of the next tile, which
+    // every wavefront reaches only after writing them)
+    auto finish = [&](int tile) {
+        const float M = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float S = 0.f, SX = 0.f, SC = 0.f;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const float f = __expf(red[w] - M);  // 0 for a wavefront with no pixel
+            S = fmaf(red[4 + w], f, S);
+            SX = fmaf(red[8 + w], f, SX);
+            SC = fmaf(red[12 + w], f, SC);
         }
+        const float inv = 1.0f / S;
+        out[2 * tile] = SX * inv;
+        out[2 * tile + 1] = 1.0f / (1.0f + __expf(-SC * inv));
     };
     load_x(blockIdx.x);
+    int prev = -1;
     for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
+        // x, channel-last on the padded grid (channel 7 and the ring stay 0):
+        // one 16-byte store per pixel
+        if (px) {
+            u32x4 v;
+            v[0] = bf16pair(xn[0], xn[1]);
+            v[1] = bf16pair(xn[2], xn[3]);
+            v[2] = bf16pair(xn[4], xn[5]);
+            v[3] = bf16pair(xn[6], 0.f);
+            *(u32x4 *)(X0 + ppx * 8) = v;
+        }
+        __syncthreads();
+        if (t == 0 && prev >= 0) finish(prev);
+        load_x(tile + gridDim.x);
+        {
+            f32x16 a0 = zero, a1 = zero;
+            nn_conv_2blocks<8, 5>(X0, A1, p0, p1, lane, a0, a1);
+            nn_store_block(H1, a0, bias[0], p0, p0 < NN_P ? nn_padded(p0) : 0, lane);
+            nn_store_block(H1, a1, bias[0], p1, p1 < NN_P ? nn_padded(p1) : 0, lane);
+        }
+        __syncthreads();
+        {
+            f32x16 a0 = zero, a1 = zero;
+            nn_conv_2blocks<NN_C, 18>(H1, A2, p0, p1, lane, a0, a1);
+            // h2 of the wave's own 64 pixels, unpadded (pixel p at row p)
+            nn_store_block(H2, a0, bias[1], p0, p0, lane);
+            nn_store_block(H2, a1, bias[1], p1, p1, lane);
+        }
+        wave_lds_sync();
+        {
+            // G of the wave's two blocks: A = h2 rows (pixel = lane % 32)
+            f32x16 g0 = zero, g1 = zero;
 #pragma unroll
-        for (int k = 0; k < XPT; k++) {
-            const int i = t + 256 * k;
-            if (i < NN_IN * NN_P) {
-                const int c = i / NN_P, p = i - c * NN_P;
-                X0[nn_padded(p) * 8 + c] = f2bf16(xn[k]);
+            for (int s = 0; s < 2; s++) {
+                const u32x4 a0 = *(const u32x4 *)(H2 + (64 * wave + co) * NN_CS + 16 * s + 8 * half);
+                const u32x4 a1 = *(const u32x4 *)(H2 + (64 * wave + 32 + co) * NN_CS + 16 * s + 8 * half);
+                g0 = nn_mfma(a0, B3[s], g0);
+                g1 = nn_mfma(a1, B3[s], g1);
+            }
+            // lane holds G[pixel 8 (v / 4) + 4 half + v % 4 of the block][column co]
+            if (co < 18) {
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3), q1 = q0 + 32;
+                    if (q0 < NN_P) Gs[q0 * NN_GS + co] = g0[v];
+                    if (q1 < NN_P) Gs[q1 * NN_GS + co] = g1[v];
+                }
             }
         }
         __syncthreads();
-        load_x(tile + gridDim.x);
-#pragma unroll
-        for (int bb = 0; bb < 2; bb++) {
-            const int p = 32 * (2 * wave + bb) + co;
-            nn_store_block(H1, nn_conv_block<8, 5>(X0, A1, p, lane, bias1), p, lane);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int bb = 0; bb < 2; bb++) {
-            const int p = 32 * (2 * wave + bb) + co;
-            nn_store_block(H2, nn_conv_block<NN_C, 18>(H1, A2, p, lane, bias2), p, lane);
-        }
-        __syncthreads();
+        // z = b3 + the 9-tap shift-add of G (neighbours outside the tile: 0)
         float z0 = b30, z1 = b31;
-        const bool px = t < NN_P;
-        if (px) {
-            const int pp = nn_padded(t);
 #pragma unroll
-            for (int r = 0; r < 9; r++) {
-                const int q = pp + (r / 3 - 1) * NN_PAD + (r % 3 - 1);
-#pragma unroll
-                for (int cb = 0; cb < 4; cb++) {
-                    const u32x4 hv = *(const u32x4 *)(H2 + q * NN_C + 8 * cb);
-                    const float4 *w0 = (const float4 *)(W3s + (0 * 9 + r) * NN_C + 8 * cb);
-                    const float4 *w1 = (const float4 *)(W3s + (1 * 9 + r) * NN_C + 8 * cb);
-                    const float4 a0 = w0[0], a1 = w0[1], c0 = w1[0], c1 = w1[1];
-                    const float h[8] = {__uint_as_float(hv[0] << 16), __uint_as_float(hv[0] & 0xffff0000u),
-                                        __uint_as_float(hv[1] << 16), __uint_as_float(hv[1] & 0xffff0000u),
-                                        __uint_as_float(hv[2] << 16), __uint_as_float(hv[2] & 0xffff0000u),
-                                        __uint_as_float(hv[3] << 16), __uint_as_float(hv[3] & 0xffff0000u)};
-                    z0 = fmaf(a0.x, h[0], fmaf(a0.y, h[1], fmaf(a0.z, h[2], fmaf(a0.w, h[3], z0))));
-                    z0 = fmaf(a1.x, h[4], fmaf(a1.y, h[5], fmaf(a1.z, h[6], fmaf(a1.w, h[7], z0))));
-                    z1 = fmaf(c0.x, h[0], fmaf(c0.y, h[1], fmaf(c0.z, h[2], fmaf(c0.w, h[3], z1))));
-                    z1 = fmaf(c1.x, h[4], fmaf(c1.y, h[5], fmaf(c1.z, h[6], fmaf(c1.w, h[7], z1))));
-                }
+        for (int r = 0; r < 9; r++) {
+            if ((nbm >> r) & 1u) {
+                const float *gq = Gs + (t + nbr_off(r)) * NN_GS;
+                z0 += gq[r];
+                z1 += gq[9 + r];
             }
         }
         // softmax statistics of the head (as head_stats) with one barrier:
         // each wavefront reduces relative to its own maximum, thread 0
         // rescales the 4 partial sums to the common one
         {
-            float m = px ? z0 : -INFINITY;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            const float m = nn_wave_max(px ? z0 : -INFINITY);
             const float e = px ? __expf(z0 - m) : 0.f;
-            float se = e, sx = e * (float)((px ? t % 15 : 0) - 7), sc = e * z1;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                se += __shfl_xor(se, o);
-                sx += __shfl_xor(sx, o);
-                sc += __shfl_xor(sc, o);
-            }
+            const float se = nn_wave_sum(e), sx = nn_wave_sum(e * (float)((px ? t % 15 : 0) - 7));
+            const float sc = nn_wave_sum(e * z1);
             if (lane == 0) {
                 red[wave] = m;
                 red[4 + wave] = se;
@@ -599,23 +708,15 @@ __global__ __launch_bounds__(256) void k_nn_infer(const uint16_t *__restrict__ p
                 red[12 + wave] = sc;
             }
         }
-        __syncthreads();
-        if (t == 0) {
-            const float M = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-            float S = 0.f, SX = 0.f, SC = 0.f;
-#pragma unroll
-            for (int w = 0; w < 4; w++) {
-                const float f = __expf(red[w] - M);  // 0 for a wavefront with no pixel
-                S = fmaf(red[4 + w], f, S);
-                SX = fmaf(red[8 + w], f, SX);
-                SC = fmaf(red[12 + w], f, SC);
-            }
-            const float inv = 1.0f / S;
-            out[2 * tile] = SX * inv;
-            out[2 * tile + 1] = 1.0f / (1.0f + __expf(-SC * inv));
-        }
-        __syncthreads();  // X0, H1, H2 and red are rewritten by the next tile
+        prev = tile;
+        // no barrier here: the next tile rewrites X0 (last read in its layer
+        // 1), H1 and H2 (layer 2) and Gs (after its second barrier, which
+        // every thread passes only after this shift-add); red is read by
+        // thread 0 after the next first barrier and rewritten only after the
+        // third
     }
+    __syncthreads();
+    if (t == 0 && prev >= 0) finish(prev);
 }
 
 // ---------------------------------------------------------------- Adam
