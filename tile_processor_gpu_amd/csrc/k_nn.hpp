@@ -424,7 +424,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int NN_PAD = 17, NN_PADP = NN_PAD * NN_PAD;
-constexpr int NN_GS = 19;  // G row pitch in floats (18 used; odd: fewer bank conflicts)
+// G row pitch in floats (18 used): odd, so the 9-tap reads of consecutive
+// pixels hit distinct banks, and 4 rows (the two half-wavefronts of a G
+// store) 84 words apart, clear of the 18 columns
+constexpr int NN_GS = 21;
 // pixel pitch of the 32-channel activations in BF16 elements: 80 bytes, so
 // the 16-byte reads of 8 consecutive pixels cover all 32 LDS banks (at 64
 // bytes they would share 8 banks, 4 lanes on each)
@@ -534,12 +537,12 @@ __device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 
     }
 }
 
-// bias + ReLU, BF16, channel-last store of this lane's 16 channels of
-// pixel p at out[idx * NN_CS]; bias: the layer's 32 biases in LDS (this lane's
-// rows 8 j + 4 (lane / 32) .. + 3 are one float4 each)
-__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, const float *bias, int p,
-                                               int idx, int lane) {
-    if (p >= NN_P) return;
+// bias + ReLU, BF16, channel-last store of this lane's 16 channels at
+// out[idx * NN_CS] (the caller sends pixels past the tile to a junk row);
+// bias: the layer's 32 biases in LDS (this lane's rows 8 j + 4 (lane / 32)
+// .. + 3 are one float4 each)
+__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, const float *bias, int idx,
+                                               int lane) {
     uint16_t *o = out + idx * NN_CS + 4 * (lane >> 5);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -575,22 +578,25 @@ __device__ __forceinline__ float nn_wave_max(float v) {
     return fmaxf(fmaxf(nn_row_lane(v, 0), nn_row_lane(v, 16)), fmaxf(nn_row_lane(v, 32), nn_row_lane(v, 48)));
 }
 
-// 2 workgroups per CU (70 KB of LDS each): 2 waves per SIMD, at most 256
+// 2 workgroups per CU (67 KB of LDS each): 2 waves per SIMD, at most 256
 // registers per lane
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, int n, float *out) {
-    __shared__ __align__(16) uint16_t X0[NN_PADP * 8];   // x, padded, channel 7 zero
-    __shared__ __align__(16) uint16_t H1[NN_PADP * NN_CS];
-    __shared__ __align__(16) uint16_t H2[256 * NN_CS];   // unpadded, rows past the tile zero
-    __shared__ __align__(16) float Gs[NN_P * NN_GS];      // G, unpadded (pixel p at row p)
+    // padded planes have one junk row after the grid (row NN_PADP), where the
+    // stores of the pixels past the tile go; unpadded ones have 256 rows
+    __shared__ __align__(16) uint16_t X0[(NN_PADP + 1) * 8];   // x, padded, channel 7 zero
+    __shared__ __align__(16) uint16_t H1[(NN_PADP + 1) * NN_CS];
+    __shared__ __align__(16) uint16_t H2[256 * NN_CS];          // unpadded (pixel p at row p)
+    __shared__ __align__(16) float Gs[256 * NN_GS];             // G, unpadded
     __shared__ __align__(16) float bias[2][NN_C];       // b1, b2
     __shared__ float red[16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int co = lane & 31, half = lane >> 5;
     if (t < 2 * NN_C) bias[t >> 5][t & 31] = bf16f(prm[(t < NN_C ? NN_OFF_B1 : NN_OFF_B2 - NN_C) + t]);
-    for (int i = t; i < NN_PADP * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
+    for (int i = t; i < (NN_PADP + 1) * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
     for (int i = t; i < 256 * NN_CS / 2; i += 256) ((uint32_t *)H2)[i] = 0u;
-    for (int i = t; i < NN_PADP * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
+    for (int i = t; i < (NN_PADP + 1) * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
+    for (int i = t; i < 256 * NN_GS; i += 256) Gs[i] = 0.f;  // finite everywhere (read under a select)
     u32x4 A1[5], A2[18], B3[2];
 #pragma unroll
     for (int kc = 0; kc < 5; kc++) A1[kc] = nn_wfrag<NN_IN>(prm + NN_OFF_W1, co, 0, 2 * kc + half, NN_IN);
@@ -614,9 +620,10 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
     const int p0 = 64 * wave + co, p1 = p0 + 32;  // this lane's pixel column in the wave's 2 blocks
     const int ppx = px ? nn_padded(t) : 0;
     const unsigned nbm = px ? nbr_mask_yx(t / 15, t % 15) : 0u;
-    // the head of the previous tile: thread 0 combines the wavefronts'
-    // partial sums in red (after the first barrier of the next tile, which
-    // every wavefront reaches only after writing them)
+    // the head of the previous tile: one thread combines the wavefronts'
+    // partial sums in red (after the second barrier of the next tile, which
+    // every wavefront reaches only after writing them; red is rewritten only
+    // after the third)
     auto finish = [&](int tile) {
         const float M = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         float S = 0.f, SX = 0.f, SC = 0.f;
@@ -636,30 +643,32 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
     for (int tile = blockIdx.x; tile < n; tile += gridDim.x) {
         // x, channel-last on the padded grid (channel 7 and the ring stay 0):
         // one 16-byte store per pixel
-        if (px) {
+        {
             u32x4 v;
             v[0] = bf16pair(xn[0], xn[1]);
             v[1] = bf16pair(xn[2], xn[3]);
             v[2] = bf16pair(xn[4], xn[5]);
             v[3] = bf16pair(xn[6], 0.f);
-            *(u32x4 *)(X0 + ppx * 8) = v;
+            *(u32x4 *)(X0 + (px ? ppx : NN_PADP) * 8) = v;
         }
         __syncthreads();
-        if (t == 0 && prev >= 0) finish(prev);
         load_x(tile + gridDim.x);
         {
             f32x16 a0 = zero, a1 = zero;
             nn_conv_2blocks<8, 5>(X0, A1, p0, p1, lane, a0, a1);
-            nn_store_block(H1, a0, bias[0], p0, p0 < NN_P ? nn_padded(p0) : 0, lane);
-            nn_store_block(H1, a1, bias[0], p1, p1 < NN_P ? nn_padded(p1) : 0, lane);
+            nn_store_block(H1, a0, bias[0], p0 < NN_P ? nn_padded(p0) : NN_PADP, lane);
+            nn_store_block(H1, a1, bias[0], p1 < NN_P ? nn_padded(p1) : NN_PADP, lane);
         }
         __syncthreads();
+        // the head of the previous tile, by the lane of the wavefront with
+        // the fewest pixels, off the path to the next barrier's other waits
+        if (t == 192 && prev >= 0) finish(prev);
         {
             f32x16 a0 = zero, a1 = zero;
             nn_conv_2blocks<NN_C, 18>(H1, A2, p0, p1, lane, a0, a1);
             // h2 of the wave's own 64 pixels, unpadded (pixel p at row p)
-            nn_store_block(H2, a0, bias[1], p0, p0, lane);
-            nn_store_block(H2, a1, bias[1], p1, p1, lane);
+            nn_store_block(H2, a0, bias[1], p0, lane);
+            nn_store_block(H2, a1, bias[1], p1, lane);
         }
         wave_lds_sync();
         {
@@ -676,9 +685,9 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
             if (co < 18) {
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
-                    const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3), q1 = q0 + 32;
-                    if (q0 < NN_P) Gs[q0 * NN_GS + co] = g0[v];
-                    if (q1 < NN_P) Gs[q1 * NN_GS + co] = g1[v];
+                    const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3);  // rows past the tile: junk
+                    Gs[q0 * NN_GS + co] = g0[v];
+                    Gs[(q0 + 32) * NN_GS + co] = g1[v];
                 }
             }
         }
@@ -687,11 +696,10 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
         float z0 = b30, z1 = b31;
 #pragma unroll
         for (int r = 0; r < 9; r++) {
-            if ((nbm >> r) & 1u) {
-                const float *gq = Gs + (t + nbr_off(r)) * NN_GS;
-                z0 += gq[r];
-                z1 += gq[9 + r];
-            }
+            const bool in = (nbm >> r) & 1u;
+            const float *gq = Gs + min(255, max(0, t + nbr_off(r))) * NN_GS;
+            z0 += in ? gq[r] : 0.f;
+            z1 += in ? gq[9 + r] : 0.f;
         }
         // softmax statistics of the head (as head_stats) with one barrier:
         // each wavefront reduces relative to its own maximum, thread 0
@@ -711,12 +719,12 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
         prev = tile;
         // no barrier here: the next tile rewrites X0 (last read in its layer
         // 1), H1 and H2 (layer 2) and Gs (after its second barrier, which
-        // every thread passes only after this shift-add); red is read by
-        // thread 0 after the next first barrier and rewritten only after the
+        // every thread passes only after this shift-add); red is read by one
+        // thread after the next second barrier and rewritten only after the
         // third
     }
     __syncthreads();
-    if (t == 0 && prev >= 0) finish(prev);
+    if (t == 192 && prev >= 0) finish(prev);
 }
 
 // ---------------------------------------------------------------- Adam
