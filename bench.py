@@ -203,7 +203,7 @@ def median_ms(fn, reps, warmup=3):
     return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
-METHODS = (("fp32", 0), ("mfma_fp16x2", 1), ("mfma_bf16x2", 2))
+METHODS = (("fp32", 0), ("mfma_fp16x2", 1), ("mfma_bf16x2", 2), ("mfma_bf16x3", 3))
 
 
 def measure_fused(cfg, fb, idx, reps, staged_ms, frame_bytes_fused, sample):

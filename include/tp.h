@@ -346,7 +346,9 @@ typedef struct tp_fused_config {
     int32_t tiles_per_item;   /* tiles per work item, a multiple of tiles_per_group (default 12) */
     int32_t tiles_per_group;  /* tiles held in LDS at once: 2 or 4 (default 4) */
     int32_t blocks_per_cu;    /* stream form: resident workgroups per CU, 1..8 (default 4) */
-    int32_t dct;              /* forward DCT-IV/DST-IV: 0 FP32 butterflies (default) */
+    int32_t dct;              /* forward DCT-IV/DST-IV: 0 FP32 butterflies (default); on the matrix
+                                 cores with split operands and FP32 accumulation: 1 FP16 hi + lo,
+                                 2 BF16 hi + lo (about 1e-4 relative), 3 BF16 hi + mid + lo */
 } tp_fused_config;
 TP_STATIC_ASSERT(sizeof(tp_fused_config) == 16, "tp_fused_config: 4 int32");
 

@@ -161,10 +161,11 @@ def test_stream_matches_single_frames(stream, cfg):
     assert run.drained()
 
 
-@pytest.mark.parametrize("dct,tol", [(1, 1e-5), (2, 1e-4)])
+@pytest.mark.parametrize("dct,tol", [(1, 1e-5), (2, 1e-4), (3, 1e-5)])
 def test_stream_mfma_matches_fp32_stream(stream, dct, tol):
     """The persistent stream with the split-precision matrix-core forward DCT
-    (1: FP16 hi + lo, 2: BF16 hi + lo), launched and replayed from a graph,
+    (1: FP16 hi + lo, 2: BF16 hi + lo, 3: BF16 hi + mid + lo), launched and
+    replayed from a graph,
     against the FP32 stream on the same frames (FP32 is within 1e-6 of the
     FP64 oracle, so the tolerance is the split form's own bound)."""
     fbs = stream["fbs"]

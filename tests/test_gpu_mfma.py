@@ -1,11 +1,12 @@
 """Split-precision matrix-core form of the forward DCT-IV/DST-IV (fused path,
-tp_fused_config.dct = 1: FP16 hi + lo, 2: BF16 hi + lo) against the FP64
-oracle, on the small configuration and its high-dynamic-range variant
-(pixels log-uniform over 1e-3 .. 1e3).
+tp_fused_config.dct = 1: FP16 hi + lo, 2: BF16 hi + lo, 3: BF16 hi + mid +
+lo) against the FP64 oracle, on the small configuration and its
+high-dynamic-range variant (pixels log-uniform over 1e-3 .. 1e3).
 
 Tolerances (max |x - x64| / max |x64| per camera and color, and per
-correlation list):  FP16 hi + lo 1e-5, as the FP32 path;  BF16 hi + lo 1e-4
-(16 significant bits: each product carries ~2^-17 relative error).
+correlation list):  FP16 hi + lo and BF16 hi + mid + lo 1e-5, as the FP32
+path;  BF16 hi + lo 1e-4 (16 significant bits: each product carries ~2^-17
+relative error; outside the 1e-5 contract, kept as the faster BF16 form).
 """
 import numpy as np
 import pytest
@@ -19,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 FZ = 0.05
 SCALES = (0.25, 0.25, 0.5)
-TOL = {1: 1e-5, 2: 1e-4}
+TOL = {1: 1e-5, 2: 1e-4, 3: 1e-5}
 
 
 def _case(hdr, seed):
@@ -42,7 +43,7 @@ def case(request, orc):
     return dict(cfg=cfg, fb=fb, clt64=clt64, corr64=corr64)
 
 
-@pytest.mark.parametrize("dct", [1, 2])
+@pytest.mark.parametrize("dct", [1, 2, 3])
 def test_mfma_dct_matches_fp64(case, dct):
     fb = case["fb"]
     for c in fb.clt:

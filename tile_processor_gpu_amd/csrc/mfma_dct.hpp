@@ -16,8 +16,15 @@
 // 16 (2 s + (i/16) / 2) + i % 16; it stores them (one float4) into the LDS
 // staging of that unit at [c][m], the layout the FP32 path writes.
 //
+// PREC 3 splits both operands into three BF16 parts, x = hi + mid + lo
+// (each part the BF16 rounding of what the previous ones leave, so the
+// parts carry 24 bits and the sum is exact to 2^-26), and keeps the six
+// products down to 2^-24: hi hi, hi mid, mid hi, mid mid, hi lo, lo hi
+// (6 instructions per product, smallest first).
+//
 // Accuracy: FP16 hi + lo carries 22 significant bits (relative error of a
-// product about 2^-22); BF16 hi + lo 16 bits (2^-16).  For FP16 each lane's
+// product about 2^-22); BF16 hi + lo 16 bits (2^-16); BF16 hi + mid + lo
+// about 24 (2^-24, the FP32 accumulation's own).  For FP16 each lane's
 // vector is first scaled by a power of two so that its largest element lies
 // in [2^14, 2^15): the hi part cannot overflow and the lo parts of all
 // elements that matter (those above 2^-11 of the largest) stay normal FP16
@@ -33,7 +40,7 @@ namespace tp {
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef short i16x4 __attribute__((ext_vector_type(4)));
 
-// PREC 1: FP16 split, 2: BF16 split
+// PREC 1: FP16 hi + lo, 2: BF16 hi + lo, 3: BF16 hi + mid + lo
 template <int PREC>
 struct SplitOps;
 
@@ -81,11 +88,34 @@ struct SplitOps<2> {
     }
 };
 
+template <>
+struct SplitOps<3> {
+    typedef i16x4 vec;
+    __device__ static __forceinline__ void split4(const float *v, vec &hi, vec &mid, vec &lo) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const short h = bf16_rne(v[e]);
+            const float r = v[e] - bf16_widen(h);  // exact
+            const short m = bf16_rne(r);
+            hi[e] = h;
+            mid[e] = m;
+            lo[e] = bf16_rne(r - bf16_widen(m));  // r - m exact
+        }
+    }
+    __device__ static __forceinline__ void split4(const float *v, vec &hi, vec &lo) {
+        vec m;
+        split4(v, hi, m, lo);
+    }
+    __device__ static __forceinline__ f32x4 mma(const vec &a, const vec &b, const f32x4 &c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+};
+
 // Constant operand of one transform for this lane: T[lane % 8][e + 4 h],
-// split, and the lane's block-diagonal masks for s = 0, 1.
+// split (mid: PREC 3 only), and the lane's block-diagonal masks for s = 0, 1.
 template <int PREC>
 struct MfmaConst {
-    typename SplitOps<PREC>::vec hi[2], lo[2];
+    typename SplitOps<PREC>::vec hi[2], mid[2], lo[2];
     bool on[2];
 };
 
@@ -103,7 +133,12 @@ __device__ __forceinline__ MfmaConst<PREC> mfma_const(int lane) {
         row[j] = PREC == 1 ? v * 256.0f : v;  // FP16: constants x 2^8 keep their lo parts normal
     }
 #pragma unroll
-    for (int h = 0; h < 2; h++) SplitOps<PREC>::split4(row + 4 * h, k.hi[h], k.lo[h]);
+    for (int h = 0; h < 2; h++) {
+        if constexpr (PREC == 3)
+            SplitOps<PREC>::split4(row + 4 * h, k.hi[h], k.mid[h], k.lo[h]);
+        else
+            SplitOps<PREC>::split4(row + 4 * h, k.hi[h], k.lo[h]);
+    }
     const int g = lane >> 4, u = lane >> 3;
 #pragma unroll
     for (int s = 0; s < 2; s++) k.on[s] = (g == 2 * s + (u & 1));
@@ -116,7 +151,7 @@ template <int PREC>
 __device__ __forceinline__ void mfma_dct8_to_lds(const float (&x)[8], const MfmaConst<PREC> &k,
                                                  float *stage, int lane, int ustride) {
     typedef typename SplitOps<PREC>::vec vec;
-    vec xh[2], xl[2];
+    vec xh[2], xm[2], xl[2];
     float inv = 1.0f;
     if constexpr (PREC == 1) {
         float mx = 0.f;
@@ -131,6 +166,9 @@ __device__ __forceinline__ void mfma_dct8_to_lds(const float (&x)[8], const Mfma
         for (int j = 0; j < 8; j++) xs[j] = x[j] * sc;
 #pragma unroll
         for (int h = 0; h < 2; h++) SplitOps<PREC>::split4(xs + 4 * h, xh[h], xl[h]);
+    } else if constexpr (PREC == 3) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) SplitOps<PREC>::split4(x + 4 * h, xh[h], xm[h], xl[h]);
     } else {
 #pragma unroll
         for (int h = 0; h < 2; h++) SplitOps<PREC>::split4(x + 4 * h, xh[h], xl[h]);
@@ -142,9 +180,19 @@ __device__ __forceinline__ void mfma_dct8_to_lds(const float (&x)[8], const Mfma
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const vec ah = k.on[s] ? k.hi[h] : zero, al = k.on[s] ? k.lo[h] : zero;
-            d = SplitOps<PREC>::mma(ah, xh[h], d);
-            d = SplitOps<PREC>::mma(al, xh[h], d);
-            d = SplitOps<PREC>::mma(ah, xl[h], d);
+            if constexpr (PREC == 3) {
+                const vec am = k.on[s] ? k.mid[h] : zero;
+                d = SplitOps<PREC>::mma(al, xh[h], d);
+                d = SplitOps<PREC>::mma(ah, xl[h], d);
+                d = SplitOps<PREC>::mma(am, xm[h], d);
+                d = SplitOps<PREC>::mma(am, xh[h], d);
+                d = SplitOps<PREC>::mma(ah, xm[h], d);
+                d = SplitOps<PREC>::mma(ah, xh[h], d);
+            } else {
+                d = SplitOps<PREC>::mma(ah, xh[h], d);
+                d = SplitOps<PREC>::mma(al, xh[h], d);
+                d = SplitOps<PREC>::mma(ah, xl[h], d);
+            }
         }
         const int grp = lane >> 4;
         const int src = 16 * (2 * s + (grp >> 1)) + (lane & 15);  // lane whose vector this is

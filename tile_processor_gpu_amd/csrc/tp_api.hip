@@ -71,7 +71,7 @@ int fused_config(const char *fn, const tp_fused_config *c, FusedCfg &o) {
     if (o.tpi < o.gt || o.tpi % o.gt || o.tpi > 4096)
         return fail(TP_EINVAL, "%s: tiles_per_item must be a multiple of tiles_per_group, <= 4096", fn);
     if (o.bpc < 1 || o.bpc > 8) return fail(TP_EINVAL, "%s: blocks_per_cu must be 1..8", fn);
-    if (o.dct < 0 || o.dct > 2) return fail(TP_EINVAL, "%s: dct must be 0, 1 or 2", fn);
+    if (o.dct < 0 || o.dct > 3) return fail(TP_EINVAL, "%s: dct must be 0..3", fn);
     return TP_OK;
 }
 
@@ -124,6 +124,7 @@ template <bool Q>
 void launch_fused(const tp::FusedArgs &a, const FusedCfg &c, int colors, int grid, hipStream_t st) {
     if (c.dct == 1) launch_fused_d<Q, 1>(a, c, colors, grid, st);
     else if (c.dct == 2) launch_fused_d<Q, 2>(a, c, colors, grid, st);
+    else if (c.dct == 3) launch_fused_d<Q, 3>(a, c, colors, grid, st);
     else launch_fused_d<Q, 0>(a, c, colors, grid, st);
 }
 
