@@ -266,8 +266,10 @@ int tp_prepare_texture_list(const tp_geometry *geom,
 
 /* Texture list + RGBA image in one stream-ordered sequence.  Replaces
  * generate_RBGA (src/TileProcessor.cuh:1446-1613): prepare_texture_list,
- * then clearing the RGBA image of the woi, then the 8 textures_accumulate
- * passes (4 tile parities x inner/border) into gpu_texture_tiles, an RGBA
+ * then clearing the RGBA image of the woi, then the textures_accumulate
+ * passes (one launch per tile parity, taking its inner and its border list;
+ * the reference runs the two lists as separate passes, which only changes
+ * the order of the overlap-add sums) into gpu_texture_tiles, an RGBA
  * image of colors + 1 slices with (woi height + 1) * 8 rows of
  * texture_rbga_stride floats (allocate (tilesy + 1) * 8 rows per slice).
  * No host synchronisation: pass sizes are read on the device.  keep_weights

@@ -31,10 +31,14 @@ struct TexArgs {
     Geo g;
     const int *woi;
     const float *const *clt;
-    int n;              // list length, or the cap of *num_ptr
-    const int *num_ptr; // device count (generate_RBGA passes)
+    int n;              // list length (quarter == 0)
+    // quarter > 0 (generate_RBGA): idx is one parity's quarter of the texture
+    // list, inner entries from its start and border entries at its end, with
+    // the device counts num_ptr[0] (inner) and num_ptr[1] (border); both are
+    // processed (tiles of one parity never overlap)
+    int quarter;
+    const int *num_ptr;
     const int *idx;
-    int border_quarter; // > 0: list is the last n entries of a quarter of this size
     const float *ports; // [4][2]
     int colors;
     float min_shot, scale_shot, diff_sigma, diff_threshold, min_agree;
@@ -290,6 +294,15 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
 }
 
+// Position in a.idx of entry e: in a parity quarter, entries past the inner
+// count are the border entries at the quarter's end.
+__device__ __forceinline__ int entry_pos(const TexArgs &a, int e) {
+    if (!a.quarter) return e;
+    const int n_in = min(a.num_ptr[0], a.quarter);
+    const int n_bd = min(a.num_ptr[1], a.quarter - n_in);
+    return e < n_in ? e : a.quarter - n_bd + (e - n_in);
+}
+
 // List entries e0 .. e0 + TPW - 1 (of n) by the whole workgroup:
 // phase 1 runs the units of all its tiles at once (8 lanes each), then the
 // threads take the tiles' pixels one tile after the other.  Entries without
@@ -298,12 +311,11 @@ template <int COLORS, int TPW>
 __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared<TPW> &sh,
                                               const TexPix &px) {
     constexpr int colors = COLORS, UPT = 4 * COLORS;  // units per tile
-    const int *list = a.border_quarter ? a.idx + (a.border_quarter - n) : a.idx;
     int code[TPW], tile[TPW];
     bool ok[TPW], any = false;
 #pragma unroll
     for (int k = 0; k < TPW; k++) {
-        code[k] = e0 + k < n ? list[e0 + k] : 0;
+        code[k] = e0 + k < n ? a.idx[entry_pos(a, e0 + k)] : 0;
         tile[k] = code[k] >> TP_CORR_NTILE_SHIFT;
         ok[k] = (code[k] & (1 << TP_LIST_TEXTURE_BIT)) && tile[k] >= 0 && tile[k] < a.g.TX * a.g.TY;
         any = any || ok[k];
@@ -344,13 +356,17 @@ __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, T
 }
 
 // A fixed grid of workgroups loops over the list (its length may only be
-// known on the device), TPW entries at a time (2 for long lists; 1 for the
-// short border lists of generate_RBGA, whose passes are one round deep).
+// known on the device), TPW entries at a time.
 template <int COLORS, int TPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
 void k_textures_accumulate(TexArgs a) {
     __shared__ __align__(16) TexShared<TPW> sh;
-    const int n = a.num_ptr ? min(*a.num_ptr, a.n) : a.n;  // device counts are capped at a.n
+    // device counts are capped at the quarter
+    int n = a.n;
+    if (a.quarter) {
+        const int n_in = min(a.num_ptr[0], a.quarter);
+        n = n_in + min(a.num_ptr[1], a.quarter - n_in);
+    }
     const TexPix px = tex_pix(threadIdx.x);
     if (threadIdx.x == 0) tex_constants(a, sh.cst);
     __syncthreads();

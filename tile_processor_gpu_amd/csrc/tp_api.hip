@@ -332,24 +332,16 @@ int tp_imclt_rbg_all(const tp_geometry *geom, float **gpu_clt, float **gpu_rbg, 
 
 namespace {
 
-// n: list length or its cap; short: a list known to be short (one entry
-// per workgroup)
-int launch_textures(const tp::TexArgs &a, int n, hipStream_t st, bool short_list = false) {
+// n: list length or its cap
+int launch_textures(const tp::TexArgs &a, int n, hipStream_t st) {
     if (n <= 0) return TP_OK;
     // workgroups loop over the list, TP_TEX_WG_PER_CU per CU
     const int cus = num_cus();
-    const int grid = std::min(n, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
-    if (short_list) {
-        if (a.colors == 3)
-            hipLaunchKernelGGL((tp::k_textures_accumulate<3, 1>), dim3(grid), dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL((tp::k_textures_accumulate<1, 1>), dim3(grid), dim3(256), 0, st, a);
-    } else {
-        if (a.colors == 3)
-            hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
-    }
+    const int grid = std::min((n + TP_TEX_TPW - 1) / TP_TEX_TPW, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
+    if (a.colors == 3)
+        hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
     return check_launch("textures_accumulate");
 }
 
@@ -362,9 +354,9 @@ tp::TexArgs tex_args(const tp_geometry *geom, const int *woi, float **clt, const
     a.woi = woi;
     a.clt = (const float *const *)clt;
     a.n = 0;
+    a.quarter = 0;
     a.num_ptr = nullptr;
     a.idx = nullptr;
-    a.border_quarter = 0;
     a.ports = ports;
     a.colors = colors;
     a.min_shot = min_shot;
@@ -522,18 +514,19 @@ int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
                              dust_remove, 0);
     a.rbga_stride = texture_rbga_stride;
     a.rbga = gpu_texture_tiles;
+    // The reference runs 8 passes, the inner then the border list of each
+    // tile parity (:1562-1610).  Tiles of one parity never overlap, inner or
+    // border, so one launch per parity takes both of its lists (inner from
+    // the start of its quarter, border grown down from the end, counts read
+    // on the device); only the order of the overlap-add sums differs.
     const int quarter = g.TX * (g.TYA >> 2);
-    for (int pass = 0; pass < 8; pass++) {  // :1562-1610
-        const int border = pass >> 2;
-        const int slot = ((pass & 3) << 1) + border;
+    for (int par = 0; par < 4; par++) {
         tp::TexArgs p = a;
-        p.num_ptr = num_texture_tiles + slot;
-        // border lists grow down from the end of their quarter: the pass
-        // reads [end - n, end) with n known only on the device
-        p.idx = gpu_texture_indices + (pass & 3) * quarter;
-        p.border_quarter = border ? quarter : 0;
+        p.num_ptr = num_texture_tiles + 2 * par;  // [inner, border]
+        p.idx = gpu_texture_indices + par * quarter;
+        p.quarter = quarter;
         p.n = quarter;
-        if ((rc = launch_textures(p, quarter, st, border != 0))) return rc;
+        if ((rc = launch_textures(p, quarter, st))) return rc;
     }
     return TP_OK;
 }
