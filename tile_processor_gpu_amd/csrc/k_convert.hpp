@@ -357,6 +357,9 @@ __device__ __forceinline__ void unit_compute(const ConvertArgs &a, const LaneFol
     // (element k from lane k).
     const int lane = __lane_id();
     float *stage_c = lds_c - (lane >> 3) * ustride, *stage_s = lds_s - (lane >> 3) * ustride;
+    // (rows written unswizzled: measured 3% faster here than the bank-spread
+    // layout of put_row8, whose extra address arithmetic costs more than the
+    // write conflicts in this kernel)
     if constexpr (DCT == 0) {
         dct4_8(f);
         if (color1) dct4_8(fs);

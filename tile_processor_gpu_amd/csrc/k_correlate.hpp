@@ -109,21 +109,17 @@ __device__ __forceinline__ void corr_finish(float (&c0)[8], float (&c1)[8], floa
     dct2_8(c2);
     dst2_8(c3);
     // transpose: row li -> LDS, read column li
-    {
-        float *q[4] = {c0, c1, c2, c3};
-#pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
-            *(float4 *)(L + qq * 72 + li * 8) = make_float4(q[qq][0], q[qq][1], q[qq][2], q[qq][3]);
-            *(float4 *)(L + qq * 72 + li * 8 + 4) = make_float4(q[qq][4], q[qq][5], q[qq][6], q[qq][7]);
-        }
-    }
+    put_row8(L, li, c0);
+    put_row8(L + 72, li, c1);
+    put_row8(L + 144, li, c2);
+    put_row8(L + 216, li, c3);
     wave_lds_sync();
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        c0[i] = L[0 * 72 + i * 8 + li];
-        c1[i] = L[1 * 72 + i * 8 + li];
-        c2[i] = L[2 * 72 + i * 8 + li];
-        c3[i] = L[3 * 72 + i * 8 + li];
+        c0[i] = get8(L, i, li);
+        c1[i] = get8(L + 72, i, li);
+        c2[i] = get8(L + 144, i, li);
+        c3[i] = get8(L + 216, i, li);
     }
     wave_lds_sync();
     // vertical DCT-II (q0, q1) / DST-II (q2, q3) on column li

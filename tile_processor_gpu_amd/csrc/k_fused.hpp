@@ -219,13 +219,10 @@ void k_fused(FusedArgs a) {
                     convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 else
                     convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
-                float *d = slice + li * 8;
-#pragma unroll
-                for (int qq = 0; qq < 4; qq++) {
-                    const float *q = qq == 0 ? q0 : qq == 1 ? q1 : qq == 2 ? q2 : q3;
-                    *(float4 *)(d + qq * 64) = make_float4(q[0], q[1], q[2], q[3]);
-                    *(float4 *)(d + qq * 64 + 4) = make_float4(q[4], q[5], q[6], q[7]);
-                }
+                put_row8(slice, li, q0);
+                put_row8(slice + 64, li, q1);
+                put_row8(slice + 128, li, q2);
+                put_row8(slice + 192, li, q3);
                 if (fr.clt) {
                     const int tile = task_tile(g, up.txy);
                     if (valid && tile >= 0)
@@ -250,10 +247,10 @@ void k_fused(FusedArgs a) {
                     if (((tk->task >> (TP_TASK_CORR_BITS + pair)) & 1) && task_tile(g, tk->txy) >= 0)
                         row = 6 * task_num + pair;
                 }
-                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * FB + li * 8;
-                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * FB + li * 8;
+                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * FB;
+                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * FB;
                 auto load = [&](int color, int which, int qq, int h) -> f32x4 {
-                    return ((const f32x4 *)((which ? tb : ta) + color * FB))[qq * 16 + h];
+                    return *(const f32x4 *)half8((which ? tb : ta) + color * FB + qq * 64, li, h);
                 };
                 corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
             }

@@ -54,14 +54,14 @@ __device__ __forceinline__ void imclt_transform(float (&q0)[8], float (&q1)[8], 
 #pragma unroll
         for (int j = 0; j < NQ; j++) {
             const float *v = q[q0i + j];
-            *(float4 *)(L + j * 72 + li * 8) = make_float4(v[0], v[1], v[2], v[3]);
-            *(float4 *)(L + j * 72 + li * 8 + 4) = make_float4(v[4], v[5], v[6], v[7]);
+            const float row[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+            put_row8(L + j * 72, li, row);
         }
         wave_lds_sync();
 #pragma unroll
         for (int j = 0; j < NQ; j++)
 #pragma unroll
-            for (int i = 0; i < 8; i++) q[q0i + j][i] = L[j * 72 + i * 8 + li];
+            for (int i = 0; i < 8; i++) q[q0i + j][i] = get8(L + j * 72, i, li);
         wave_lds_sync();
     }
     dct4_8(q0);

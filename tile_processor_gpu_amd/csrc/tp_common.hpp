@@ -206,6 +206,24 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// An 8 x 8 block in LDS written one row per lane (two float4 halves) and
+// read back one column per lane.  Rows 4..7 have their halves swapped: at
+// the 32-byte row stride the 8 lanes' 16-byte writes would otherwise use
+// half of the banks (a 2-way conflict on every write); with the swap they
+// cover all of them.  Element (row, col) is at row * 8 + (col ^ (row & 4)).
+__device__ __forceinline__ void put_row8(float *blk, int row, const float (&v)[8]) {
+    const int s = row & 4;
+    *(float4 *)(blk + row * 8 + s) = make_float4(v[0], v[1], v[2], v[3]);
+    *(float4 *)(blk + row * 8 + (s ^ 4)) = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ float get8(const float *blk, int row, int col) {
+    return blk[row * 8 + (col ^ (row & 4))];
+}
+// float4 half h (columns 4 h .. 4 h + 3) of row `row`
+__device__ __forceinline__ const float *half8(const float *blk, int row, int h) {
+    return blk + row * 8 + ((4 * h) ^ (row & 4));
+}
+
 // DPP: value of lane (lane ^ 7), i.e. mirror within each group of 8 lanes
 __device__ __forceinline__ float mirror8(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141 /* row_half_mirror */,
