@@ -461,6 +461,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the plain staged path (no HBM ceilings, fused or stream measurements)")
     ap.add_argument("--stream-frames", type=int, default=64)
+    ap.add_argument("--dct", type=int, choices=[0, 1, 2, 3], default=0,
+                    help="forward-DCT method of --mode fused: 0 FP32, 1 FP16 hi+lo, 2 BF16 hi+lo, "
+                         "3 BF16 hi+mid+lo on the matrix cores")
     ap.add_argument("--mode", choices=["frame", "copy", "fused", "texture", "nn"], default="frame",
                     help="copy: device-to-device copies of 1 GiB (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against known bytes); fused: the fused kernel "
@@ -491,8 +494,10 @@ def main():
 
     if args.mode == "fused":  # the fused kernel alone (default launch config), for profiling
         cfg, fb, idx, _ = setup_frame(args.seed)
-        ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO), max(1, args.steps), args.warmup)
-        print(json.dumps({"mode": "fused", "ms": round(ms, 4), "steps": args.steps}), flush=True)
+        c = tp.FusedConfig(dct=args.dct)
+        ms = median_ms(lambda: fb.fused(3, SCALES, FAT_ZERO, config=c), max(1, args.steps), args.warmup)
+        print(json.dumps({"mode": "fused", "dct": args.dct, "ms": round(ms, 4), "steps": args.steps}),
+              flush=True)
         return
 
     if args.mode == "nn":  # network inference alone on the full frame's tiles, for profiling

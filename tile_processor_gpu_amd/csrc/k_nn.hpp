@@ -410,30 +410,43 @@ __global__ __launch_bounds__(256) void k_nn_backward(NNBackArgs a) {
 //     G, zero outside the tile: h2 is zero there).
 // The head (softmax, disparity, confidence) runs on the vector units.
 //
-// Activations live in LDS channel-last, the inputs of the 3x3 convolutions
-// (x, h1) and G on a zero-padded 17 x 17 grid (pixel (y, x) at (y + 1) * 17
-// + x + 1), h2 unpadded (the 1x1 product reads only a wavefront's own
-// pixels), so the 8 consecutive channels a lane contributes to one K slice
-// are one 16-byte load.  Operand layouts (32x32x16): lane i holds
+// Activations live in LDS channel-last, so the 8 consecutive channels a
+// lane contributes to one K slice are one 16-byte load.  The MFMA columns
+// are 256 pixel slots, slot s = pixel (y, x) = (s / 16, s % 16): slots with
+// x = 15 or y = 15 are past the tile (computed, never stored).  The inputs
+// of the 3x3 convolutions (x, h1) lie on a padded grid of 16 positions per
+// row, slot s at position s + 17 (NN_POFF): column 15 of row y is also
+// column -1 of row y + 1, and rows -1 and 15 stay zero, so the neighbour r
+// of every slot is at position s + 17 + 16 (r / 3 - 1) + r % 3 - 1, in 0..289.
+// h2 and G are held by slot.  Operand layouts (32x32x16): lane i holds
 // A[i % 32][8 (i / 32) + e], B[8 (i / 32) + e][i % 32] (e = 0..7) and
 // D[8 (v / 4) + 4 (i / 32) + v % 4][i % 32] (v = 0..15).  Each wavefront
-// takes 2 of a tile's 8 blocks of 32 pixels and runs their two MFMA chains
+// takes 2 of a tile's 8 blocks of 32 slots and runs their two MFMA chains
 // interleaved (independent accumulators); the weights stay in registers.
+//
+// LDS banks (gfx950, measured with the bank-conflict counter): a 16-byte
+// read serves each half-wavefront in one pass when lanes 0..15 of the half
+// hit 16 distinct 16-byte bank groups (address / 16 mod 16) and lane j + 16
+// hits the same group as lane j; a 16-byte write serves 8 lanes per pass
+// (groups address / 16 mod 8); an 8-byte write 16 lanes (address / 8 mod
+// 16); 4-byte accesses 32 lanes (address / 4 mod 32).  The layouts below
+// keep every access of the loop conflict-free on these rules.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int NN_PAD = 17, NN_PADP = NN_PAD * NN_PAD;
-// G row pitch in floats (18 used): odd, so the 9-tap reads of consecutive
-// pixels hit distinct banks, and 4 rows (the two half-wavefronts of a G
-// store) 84 words apart, clear of the 18 columns
+// padded grid: 16 positions per row, slot s at s + NN_POFF; positions past
+// the last one read (289) hold the stores of the slots past the tile
+constexpr int NN_PW = 16, NN_POFF = NN_PW + 1, NN_PJUNK = 290, NN_PROWS = NN_PJUNK + 1;
+// G row pitch in floats (18 used): odd, so the 9-tap reads of 32
+// consecutive slots fall in 32 distinct banks
 constexpr int NN_GS = 21;
-// pixel pitch of the 32-channel activations in BF16 elements: 80 bytes, so
-// the 16-byte reads of 8 consecutive pixels cover all 32 LDS banks (at 64
-// bytes they would share 8 banks, 4 lanes on each)
+// position pitch of the 32-channel activations in BF16 elements: 80 bytes,
+// 5 bank groups (odd), so 16 consecutive positions fall in 16 distinct
+// groups and positions 16 apart (lanes j and j + 16) in the same one
 constexpr int NN_CS = 40;
 
-__device__ __forceinline__ int nn_padded(int p) { return (p / 15 + 1) * NN_PAD + p % 15 + 1; }
+__device__ __forceinline__ bool nn_slot_in_tile(int s) { return (s & 15) < 15 && s < 15 * NN_PW; }
 
 // A fragment: weights W[co][ci0 + e][r] of layer (CIN inputs), e = 0..7
 // (channels >= cin_valid or r >= 9 give 0)
@@ -491,20 +504,19 @@ __device__ __forceinline__ f32x16 nn_mfma(const u32x4 &a, const u32x4 &b, const 
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-// Two 32-pixel blocks (pixels p0, p1 of this lane's column) of a 3x3
-// convolution: in = padded LDS activations with C channels per pixel (pitch
-// 8 for C = 8, NN_CS for C = 32); NK chunks of 16 K values; A = weight
-// fragments.  k ordering per chunk kc: C = 8: r = 2 kc + i / 32, channels
-// 0..7; C = 32: r = kc / 2, channels (kc % 2) 16 + 8 (i / 32) .. + 7.
-// Pixels past the tile read the zero ring.  The B fragments of chunk kc + 2
-// are requested while chunks kc and kc + 1 are multiplied (the LDS latency
-// is not covered by other wavefronts at 2 waves per SIMD).
+// Two 32-slot blocks (slots s0, s1 of this lane's column) of a 3x3
+// convolution: in = padded LDS activations with C channels per position
+// (pitch 8 for C = 8, NN_CS for C = 32); NK chunks of 16 K values; A =
+// weight fragments.  k ordering per chunk kc: C = 8: r = 2 kc + i / 32,
+// channels 0..7; C = 32: r = kc / 2, channels (kc % 2) 16 + 8 (i / 32) .. + 7.
+// The B fragments of chunk kc + 2 are requested while chunks kc and kc + 1
+// are multiplied (the LDS latency is not covered by other wavefronts at 2
+// waves per SIMD).
 template <int C, int NK>
-__device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 (&A)[NK], int p0, int p1,
+__device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 (&A)[NK], int s0, int s1,
                                                 int lane, f32x16 &acc0, f32x16 &acc1) {
     constexpr int CS = C == 8 ? 8 : NN_CS;
     constexpr int D = 2;  // prefetch distance in chunks
-    const int pp0 = p0 < NN_P ? nn_padded(p0) : -1, pp1 = p1 < NN_P ? nn_padded(p1) : -1;
     const int half = lane >> 5;
     auto fetch = [&](int kc, u32x4 &b0, u32x4 &b1) {
         int r, c0;
@@ -518,9 +530,9 @@ __device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 
         b0 = u32x4{0u, 0u, 0u, 0u};
         b1 = u32x4{0u, 0u, 0u, 0u};
         if (r < 9) {
-            const int d = (r / 3 - 1) * NN_PAD + (r % 3 - 1);
-            b0 = *(const u32x4 *)(in + (pp0 >= 0 ? pp0 + d : 0) * CS + c0);
-            b1 = *(const u32x4 *)(in + (pp1 >= 0 ? pp1 + d : 0) * CS + c0);
+            const int d = NN_POFF + (r / 3 - 1) * NN_PW + (r % 3 - 1);
+            b0 = *(const u32x4 *)(in + (s0 + d) * CS + c0);
+            b1 = *(const u32x4 *)(in + (s1 + d) * CS + c0);
         }
     };
     u32x4 b0[D + 1], b1[D + 1];
@@ -537,20 +549,36 @@ __device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 
     }
 }
 
-// bias + ReLU, BF16, channel-last store of this lane's 16 channels at
-// out[idx * NN_CS] (the caller sends pixels past the tile to a junk row);
-// bias: the layer's 32 biases in LDS (this lane's rows 8 j + 4 (lane / 32)
-// .. + 3 are one float4 each)
+// bias + ReLU, BF16, channel-last store of one 32-slot block at
+// out[idx * NN_CS] (the caller sends slots past the tile to a junk
+// position).  Lane i holds channels 8 j + 4 (i / 32) .. + 3 (j = 0..3) of
+// its slot; the two wavefront halves swap half their words
+// (v_permlane32_swap), after which the lower half holds channels 0..7 and
+// 16..23 and the upper half 8..15 and 24..31, each run stored as one
+// 16-byte write.  bias: the layer's 32 biases in LDS (this lane's rows
+// 8 j + 4 (i / 32) .. + 3 are one float4 each).
 __device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, const float *bias, int idx,
                                                int lane) {
-    uint16_t *o = out + idx * NN_CS + 4 * (lane >> 5);
+    const int half = lane >> 5;
+    uint32_t w[8];  // w[2 j], w[2 j + 1]: channels 8 j + 4 half + (0, 1), (2, 3)
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const float4 b = *(const float4 *)(bias + 8 * j + 4 * (lane >> 5));
-        const uint32_t lo = relu_bf16x2(bf16pair(acc[4 * j] + b.x, acc[4 * j + 1] + b.y));
-        const uint32_t hi = relu_bf16x2(bf16pair(acc[4 * j + 2] + b.z, acc[4 * j + 3] + b.w));
-        *(uint2 *)(o + 8 * j) = make_uint2(lo, hi);
+        const float4 b = *(const float4 *)(bias + 8 * j + 4 * half);
+        w[2 * j] = relu_bf16x2(bf16pair(acc[4 * j] + b.x, acc[4 * j + 1] + b.y));
+        w[2 * j + 1] = relu_bf16x2(bf16pair(acc[4 * j + 2] + b.z, acc[4 * j + 3] + b.w));
     }
+    // swap(x, y): lower half keeps x and gets the upper half's x; upper half
+    // gets the lower half's y and keeps y
+    u32x4 run[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const auto r0 = __builtin_amdgcn_permlane32_swap(w[4 * c], w[4 * c + 2], false, false);
+        const auto r1 = __builtin_amdgcn_permlane32_swap(w[4 * c + 1], w[4 * c + 3], false, false);
+        run[c] = u32x4{r0[0], r1[0], r0[1], r1[1]};
+    }
+    uint16_t *o = out + idx * NN_CS + 8 * half;
+    *(u32x4 *)o = run[0];
+    *(u32x4 *)(o + 16) = run[1];
 }
 
 // All-lane sum / max of a wavefront: butterfly within each row of 16 lanes
@@ -578,24 +606,22 @@ __device__ __forceinline__ float nn_wave_max(float v) {
     return fmaxf(fmaxf(nn_row_lane(v, 0), nn_row_lane(v, 16)), fmaxf(nn_row_lane(v, 32), nn_row_lane(v, 48)));
 }
 
-// 2 workgroups per CU (67 KB of LDS each): 2 waves per SIMD, at most 256
+// 2 workgroups per CU (69 KB of LDS each): 2 waves per SIMD, at most 256
 // registers per lane
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, int n, float *out) {
-    // padded planes have one junk row after the grid (row NN_PADP), where the
-    // stores of the pixels past the tile go; unpadded ones have 256 rows
-    __shared__ __align__(16) uint16_t X0[(NN_PADP + 1) * 8];   // x, padded, channel 7 zero
-    __shared__ __align__(16) uint16_t H1[(NN_PADP + 1) * NN_CS];
-    __shared__ __align__(16) uint16_t H2[256 * NN_CS];          // unpadded (pixel p at row p)
-    __shared__ __align__(16) float Gs[256 * NN_GS];             // G, unpadded
+    __shared__ __align__(16) uint16_t X0[NN_PROWS * 8];   // x, padded, channel 7 zero
+    __shared__ __align__(16) uint16_t H1[NN_PROWS * NN_CS];
+    __shared__ __align__(16) uint16_t H2[256 * NN_CS];    // by slot
+    __shared__ __align__(16) float Gs[256 * NN_GS];       // by slot
     __shared__ __align__(16) float bias[2][NN_C];       // b1, b2
     __shared__ float red[16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int co = lane & 31, half = lane >> 5;
     if (t < 2 * NN_C) bias[t >> 5][t & 31] = bf16f(prm[(t < NN_C ? NN_OFF_B1 : NN_OFF_B2 - NN_C) + t]);
-    for (int i = t; i < (NN_PADP + 1) * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
+    for (int i = t; i < NN_PROWS * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
     for (int i = t; i < 256 * NN_CS / 2; i += 256) ((uint32_t *)H2)[i] = 0u;
-    for (int i = t; i < (NN_PADP + 1) * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
+    for (int i = t; i < NN_PROWS * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
     for (int i = t; i < 256 * NN_GS; i += 256) Gs[i] = 0.f;  // finite everywhere (read under a select)
     u32x4 A1[5], A2[18], B3[2];
 #pragma unroll
@@ -607,19 +633,22 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
     B3[1] = nn_w3frag(prm, lane, 1);
     const float b30 = bf16f(prm[NN_OFF_B3]), b31 = bf16f(prm[NN_OFF_B3 + 1]);
     __syncthreads();
-    // thread t = pixel t: its 7 input channels of a tile, requested one tile
-    // ahead
-    const bool px = t < NN_P;
+    // thread t = pixel t of the input (row-major 15 x 15): its 7 input
+    // channels of a tile, requested one tile ahead
+    const bool pin = t < NN_P;
     float xn[NN_IN];
     auto load_x = [&](int tile) {
-        const float *xt = x + (size_t)tile * NN_IN * NN_P + (px ? t : 0);
+        const float *xt = x + (size_t)tile * NN_IN * NN_P + (pin ? t : 0);
 #pragma unroll
-        for (int c = 0; c < NN_IN; c++) xn[c] = (tile < n && px) ? xt[c * NN_P] : 0.f;
+        for (int c = 0; c < NN_IN; c++) xn[c] = (tile < n && pin) ? xt[c * NN_P] : 0.f;
     };
+    const int xpos = pin ? NN_PW * (t / 15) + t % 15 + NN_POFF : NN_PJUNK;
     const f32x16 zero = {};
-    const int p0 = 64 * wave + co, p1 = p0 + 32;  // this lane's pixel column in the wave's 2 blocks
-    const int ppx = px ? nn_padded(t) : 0;
-    const unsigned nbm = px ? nbr_mask_yx(t / 15, t % 15) : 0u;
+    const int s0 = 64 * wave + co, s1 = s0 + 32;  // this lane's slot column in the wave's 2 blocks
+    // thread t = slot t in the head
+    const int sy = t >> 4, sx = t & 15;
+    const bool px = nn_slot_in_tile(t);
+    const unsigned nbm = px ? nbr_mask_yx(sy, sx) : 0u;
     // the head of the previous tile: one thread combines the wavefronts'
     // partial sums in red (after the second barrier of the next tile, which
     // every wavefront reaches only after writing them; red is rewritten only
@@ -649,15 +678,15 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
             v[1] = bf16pair(xn[2], xn[3]);
             v[2] = bf16pair(xn[4], xn[5]);
             v[3] = bf16pair(xn[6], 0.f);
-            *(u32x4 *)(X0 + (px ? ppx : NN_PADP) * 8) = v;
+            *(u32x4 *)(X0 + xpos * 8) = v;
         }
         __syncthreads();
         load_x(tile + gridDim.x);
         {
             f32x16 a0 = zero, a1 = zero;
-            nn_conv_2blocks<8, 5>(X0, A1, p0, p1, lane, a0, a1);
-            nn_store_block(H1, a0, bias[0], p0 < NN_P ? nn_padded(p0) : NN_PADP, lane);
-            nn_store_block(H1, a1, bias[0], p1 < NN_P ? nn_padded(p1) : NN_PADP, lane);
+            nn_conv_2blocks<8, 5>(X0, A1, s0, s1, lane, a0, a1);
+            nn_store_block(H1, a0, bias[0], nn_slot_in_tile(s0) ? s0 + NN_POFF : NN_PJUNK, lane);
+            nn_store_block(H1, a1, bias[0], nn_slot_in_tile(s1) ? s1 + NN_POFF : NN_PJUNK, lane);
         }
         __syncthreads();
         // the head of the previous tile, by the lane of the wavefront with
@@ -665,14 +694,14 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
         if (t == 192 && prev >= 0) finish(prev);
         {
             f32x16 a0 = zero, a1 = zero;
-            nn_conv_2blocks<NN_C, 18>(H1, A2, p0, p1, lane, a0, a1);
-            // h2 of the wave's own 64 pixels, unpadded (pixel p at row p)
-            nn_store_block(H2, a0, bias[1], p0, lane);
-            nn_store_block(H2, a1, bias[1], p1, lane);
+            nn_conv_2blocks<NN_C, 18>(H1, A2, s0, s1, lane, a0, a1);
+            // h2 of the wave's own 64 slots
+            nn_store_block(H2, a0, bias[1], s0, lane);
+            nn_store_block(H2, a1, bias[1], s1, lane);
         }
         wave_lds_sync();
         {
-            // G of the wave's two blocks: A = h2 rows (pixel = lane % 32)
+            // G of the wave's two blocks: A = h2 rows (slot = lane % 32)
             f32x16 g0 = zero, g1 = zero;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
@@ -681,11 +710,11 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
                 g0 = nn_mfma(a0, B3[s], g0);
                 g1 = nn_mfma(a1, B3[s], g1);
             }
-            // lane holds G[pixel 8 (v / 4) + 4 half + v % 4 of the block][column co]
+            // lane holds G[slot 8 (v / 4) + 4 half + v % 4 of the block][column co]
             if (co < 18) {
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
-                    const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3);  // rows past the tile: junk
+                    const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3);  // slots past the tile: junk
                     Gs[q0 * NN_GS + co] = g0[v];
                     Gs[(q0 + 32) * NN_GS + co] = g1[v];
                 }
@@ -697,7 +726,7 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
 #pragma unroll
         for (int r = 0; r < 9; r++) {
             const bool in = (nbm >> r) & 1u;
-            const float *gq = Gs + min(255, max(0, t + nbr_off(r))) * NN_GS;
+            const float *gq = Gs + min(255, max(0, t + NN_PW * (r / 3 - 1) + r % 3 - 1)) * NN_GS;
             z0 += in ? gq[r] : 0.f;
             z1 += in ? gq[9 + r] : 0.f;
         }
@@ -707,12 +736,12 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
         {
             const float m = nn_wave_max(px ? z0 : -INFINITY);
             const float e = px ? __expf(z0 - m) : 0.f;
-            const float se = nn_wave_sum(e), sx = nn_wave_sum(e * (float)((px ? t % 15 : 0) - 7));
+            const float se = nn_wave_sum(e), sxe = nn_wave_sum(e * (float)(sx - 7));
             const float sc = nn_wave_sum(e * z1);
             if (lane == 0) {
                 red[wave] = m;
                 red[4 + wave] = se;
-                red[8 + wave] = sx;
+                red[8 + wave] = sxe;
                 red[12 + wave] = sc;
             }
         }
