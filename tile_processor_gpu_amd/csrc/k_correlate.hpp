@@ -17,6 +17,11 @@
 //   4. unfold to (2r+1)^2: column j combines its own column with column j-1
 //      (one DPP row shift) into output columns r+j and r-j, staged in LDS;
 //      the wavefront then stores its 8 output rows with contiguous stores.
+// LDS per wavefront: unit u's transposition blocks at CORR_UNIT_STRIDE u
+// (296 floats, 8 banks apart: the column reads of 4 units in one 32-lane
+// pass fall in 32 distinct banks), then the 8 staged 15x15 tiles
+// contiguous at 225 u (the 16-byte reads of the output block are aligned
+// and conflict-free).
 #pragma once
 
 #include "tp_common.hpp"
@@ -93,10 +98,12 @@ __device__ __forceinline__ void corr_accumulate(Load load, const float *scale, i
 // Normalise, back-transform and unfold one entry (row li of its 8 lanes):
 // amplitude normalisation with fat zero and lpf_corr (:2915-2943,
 // :1337-1347), horizontal DCT-II/DST-II, transpose through L, vertical
-// DCT-II/DST-II, unfold (:2950-3008) into the full 15x15 tile staged in L
-// (row-major, centre (7, 7)).  L: 4 * 72 floats private to the entry.
+// DCT-II/DST-II, unfold (:2950-3008) into the full 15x15 tile staged at T
+// (row-major, centre (7, 7)).  L: 4 * 72 floats private to the entry; T:
+// 225 floats, written after every lane of the wavefront has read its L (T
+// may overlap other entries' L).
 __device__ __forceinline__ void corr_finish(float (&c0)[8], float (&c1)[8], float (&c2)[8],
-                                            float (&c3)[8], float *L, int li, float fz2) {
+                                            float (&c3)[8], float *L, float *T, int li, float fz2) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const float s2 = fmaf(c3[k], c3[k], fmaf(c2[k], c2[k], fmaf(c1[k], c1[k], fmaf(c0[k], c0[k], fz2))));
@@ -143,10 +150,10 @@ __device__ __forceinline__ void corr_finish(float (&c0)[8], float (&c1)[8], floa
         const float a2 = i ? c2[i - 1] : 0.f;
         const float a3 = (i && j) ? p3[i - 1] : 0.f;
         const float t1 = a0 + a2, t2 = a0 - a2, u1 = a1 + a3, u2 = a1 - a3;
-        L[(7 + i) * 15 + 7 + j] = s4 * (t1 + u1);
-        L[(7 + i) * 15 + 7 - j] = s4 * (t1 - u1);
-        L[(7 - i) * 15 + 7 + j] = s4 * (t2 + u2);
-        L[(7 - i) * 15 + 7 - j] = s4 * (t2 - u2);
+        T[(7 + i) * 15 + 7 + j] = s4 * (t1 + u1);
+        T[(7 + i) * 15 + 7 - j] = s4 * (t1 - u1);
+        T[(7 - i) * 15 + 7 + j] = s4 * (t2 + u2);
+        T[(7 - i) * 15 + 7 - j] = s4 * (t2 - u2);
     }
 }
 
@@ -168,10 +175,13 @@ __device__ __forceinline__ int pair_cam2(int pair) {
     return pair == 0 ? 1 : pair == 1 ? 3 : pair == 2 ? 2 : pair == 3 ? 3 : pair == 4 ? 3 : 1;
 }
 
+constexpr int CORR_UNIT_STRIDE = 296;
+
 template <int COLORS>
 __global__ __launch_bounds__(64 * TP_CORR_WG_WAVES) __attribute__((amdgpu_waves_per_eu(TP_CORR_WAVES)))
 void k_correlate2D(CorrArgs a) {
-    __shared__ __align__(16) float lds[TP_CORR_WG_WAVES][8][4][72];  // [wave][unit][quadrant][8x8+pad]
+    static_assert(8 * CORR_UNIT_STRIDE >= 8 * 225, "staging fits the wavefront's region");
+    __shared__ __align__(16) float lds[TP_CORR_WG_WAVES][8 * CORR_UNIT_STRIDE];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
@@ -210,21 +220,16 @@ void k_correlate2D(CorrArgs a) {
         };
         float c0[8], c1[8], c2[8], c3[8];
         corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
-        float *L = &lds[wave][unit][0][0];
-        corr_finish(c0, c1, c2, c3, L, li, fz2);
+        float *W = &lds[wave][0];
+        corr_finish(c0, c1, c2, c3, W + CORR_UNIT_STRIDE * unit, W + 225 * unit, li, fz2);
         wave_lds_sync();
         const int vflag = valid ? 1 : 0;
         if (dense && __all(vflag)) {
             // 8 consecutive full 15 x 15 rows, contiguous in memory (7200 B,
-            // 16-byte aligned): float4 q of the block from the staged tiles
+            // 16-byte aligned) and in LDS: float4 q of the block
             float *dst = a.corrs + (size_t)eb * 225;
             for (int q = lane; q < 450; q += 64) {
-                float v[4];
-#pragma unroll
-                for (int e4 = 0; e4 < 4; e4++) {
-                    const int f = 4 * q + e4, u = f / 225;
-                    v[e4] = (&lds[wave][u][0][0])[f - 225 * u];  // staged tile u, element f - 225 u
-                }
+                const f32x4 v = *(const f32x4 *)(W + 4 * q);
                 store4(dst + 4 * q, v[0], v[1], v[2], v[3]);
             }
             wave_lds_sync();
@@ -232,7 +237,7 @@ void k_correlate2D(CorrArgs a) {
         }
         for (int u = 0; u < 8; u++) {
             if (!__shfl(vflag, u * 8)) continue;
-            const float *src = &lds[wave][u][0][0];
+            const float *src = W + 225 * u;
             float *dst = a.corrs + (size_t)(eb + u) * a.stride;
 #pragma unroll
             for (int t = 0; t < 4; t++)

@@ -10,12 +10,29 @@
 // Workgroup: 3 wavefronts (one per color) of 8 units x 8 lanes, as in
 // k_convert_correct_tiles.  Per group of GT tiles:
 //   1. convert: each wavefront converts its color for GT x 4 (tile, camera)
-//      units (GT / 2 passes of 8 units) into clt_lds[GT][4][3][256];
+//      units (GT / 2 passes of 8 units) into the CLT region of LDS;
 //   2. barrier; correlate: 6 GT (tile, pair) slots, 8 per wavefront, each
 //      accumulates its bicomplex product from LDS;
 //   3. barrier; the CLT region is reused as per-slot scratch for the
 //      normalisation, DCT-II back-transform and unfold; rows are stored;
 //   4. barrier before the next group overwrites LDS.
+//
+// LDS layout (bank rules of gfx950: a 16-byte read pass is 16 lanes, which
+// must hit 16 distinct 16-byte bank groups, address / 16 mod 16, with lane
+// j + 16 of a half-wavefront on the group of lane j; a 4-byte pass is 32
+// lanes on 32 banks):
+//   - CLT block of (tile, camera, color): 256 floats at
+//     ((tile * 4 + camera) * 3 + color) * 256, rows as put_row8 writes them
+//     for cameras 0 and 3 (halves of rows 4..7 swapped) and with the halves
+//     of rows 0..3 swapped for cameras 1 and 2: a half-row read of the 8
+//     rows covers 8 bank groups, the complementary 8 for the other kind.
+//   - Correlation slots: per tile pairs 0..3 first (slots 4 tile + pair),
+//     then pairs 4, 5 of all tiles (slots 4 GT + 2 tile + pair - 4).  The
+//     two slots of each 16-lane group then read blocks of opposite kinds,
+//     and lanes j and j + 16 blocks of the same kind, in every one of the
+//     48 loads.
+//   - Slot scratch (normalisation, back-transform, staged output tile): 296
+//     floats per slot, so the 4 slots of a 32-lane pass fall 8 banks apart.
 //
 // Work items are runs of tiles_per_item consecutive tiles (in the visiting
 // order of swizzle_strip) of one frame.  Single-frame launches map one
@@ -90,8 +107,17 @@ void k_fused(FusedArgs a) {
     constexpr int NSLOT = 6 * GT;
     // [tile][camera][color][4 x 8 x 8]; a unit's slice also serves as its
     // transposition staging (2 x 72 floats) before its CLT is written there
-    constexpr int FB = 256;
-    __shared__ __align__(16) float clt_lds[GT * 4 * 3 * FB];
+    constexpr int FB = 256, CLT_TILE = 12 * FB, SLOT_STRIDE = 296;
+    static_assert(NSLOT * SLOT_STRIDE <= GT * CLT_TILE, "slot scratch fits the CLT region");
+    __shared__ __align__(16) float clt_lds[GT * CLT_TILE];
+    auto cam_base = [](int c) { return c * 3 * FB; };
+    // which row halves a camera's blocks swap: 4 (rows 4..7) or 0 (rows 0..3)
+    // is XORed into (row & 4) ^ 4 h; cameras 1 and 2 swap rows 0..3
+    auto cam_swap = [](int c) { return ((c ^ (c >> 1)) & 1) << 2; };
+    // slot -> (tile, pair) and back
+    auto slot_tile = [](int sl) { return sl < 4 * GT ? sl >> 2 : (sl - 4 * GT) >> 1; };
+    auto slot_pair = [](int sl) { return sl < 4 * GT ? sl & 3 : 4 + ((sl - 4 * GT) & 1); };
+    auto pair_slot = [](int tl, int pr) { return pr < 4 ? 4 * tl + pr : 4 * GT + 2 * tl + pr - 4; };
     __shared__ int s_item, s_frame;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -212,17 +238,24 @@ void k_fused(FusedArgs a) {
                 else unit_extra<2>(ca, cam, up);
                 valid = valid && up.task != 0;
                 float q0[8], q1[8], q2[8], q3[8];
-                float *slice = clt_lds + ((tl * 4 + cam) * 3 + color) * FB;  // units 3 FB floats apart
+                float *slice = clt_lds + tl * CLT_TILE + cam_base(cam) + color * FB;
                 if (color == 0)
                     convert_unit<0, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 else if (color == 1)
                     convert_unit<1, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
                 else
                     convert_unit<2, DCT>(ca, lf, slice, slice + 72, li, cam, task_num, q0, q1, q2, q3, mt, 3 * FB, &up);
-                put_row8(slice, li, q0);
-                put_row8(slice + 64, li, q1);
-                put_row8(slice + 128, li, q2);
-                put_row8(slice + 192, li, q3);
+                {
+                    const int sw = (li & 4) ^ cam_swap(cam);  // offset of the row's first half
+                    float *r = slice + li * 8;
+                    const float(*qv[4])[8] = {&q0, &q1, &q2, &q3};
+#pragma unroll
+                    for (int qq = 0; qq < 4; qq++) {
+                        const float(&v)[8] = *qv[qq];
+                        *(float4 *)(r + qq * 64 + sw) = make_float4(v[0], v[1], v[2], v[3]);
+                        *(float4 *)(r + qq * 64 + (sw ^ 4)) = make_float4(v[4], v[5], v[6], v[7]);
+                    }
+                }
                 if (fr.clt) {
                     const int tile = task_tile(g, up.txy);
                     if (valid && tile >= 0)
@@ -238,8 +271,10 @@ void k_fused(FusedArgs a) {
             int row = -1;
             float c0[8], c1[8], c2[8], c3[8];
             if (active) {
-                const int tl = slot < NSLOT ? slot / 6 : 0;
-                const int pair = slot < NSLOT ? slot - 6 * tl : 0;
+                // slots past the last one (GT = 2) read pairs 0 and 1 of
+                // tile 0, in the offset pattern of real slots
+                const int tl = slot < NSLOT ? slot_tile(slot) : 0;
+                const int pair = slot < NSLOT ? slot_pair(slot) : slot & 1;
                 const int s = gs + tl;
                 if (slot < NSLOT && s < s1) {
                     const int task_num = swizzle_strip(s, g.TX, fr.num_tiles);
@@ -247,10 +282,12 @@ void k_fused(FusedArgs a) {
                     if (((tk->task >> (TP_TASK_CORR_BITS + pair)) & 1) && task_tile(g, tk->txy) >= 0)
                         row = 6 * task_num + pair;
                 }
-                const float *ta = clt_lds + (tl * 4 + pair_cam1(pair)) * 3 * FB;
-                const float *tb = clt_lds + (tl * 4 + pair_cam2(pair)) * 3 * FB;
+                const int ca1 = pair_cam1(pair), ca2 = pair_cam2(pair);
+                const float *ta = clt_lds + tl * CLT_TILE + cam_base(ca1) + li * 8;
+                const float *tb = clt_lds + tl * CLT_TILE + cam_base(ca2) + li * 8;
+                const int xa = (li & 4) ^ cam_swap(ca1), xb = (li & 4) ^ cam_swap(ca2);
                 auto load = [&](int color, int which, int qq, int h) -> f32x4 {
-                    return *(const f32x4 *)half8((which ? tb : ta) + color * FB + qq * 64, li, h);
+                    return *(const f32x4 *)((which ? tb : ta) + color * FB + qq * 64 + ((4 * h) ^ (which ? xb : xa)));
                 };
                 corr_accumulate<COLORS>(load, a.scale, li, c0, c1, c2, c3);
             }
@@ -258,14 +295,14 @@ void k_fused(FusedArgs a) {
 
             // (3) normalise, back-transform, unfold in the (now free) CLT region
             if (active) {
-                float *L = clt_lds + slot * 288;
-                corr_finish(c0, c1, c2, c3, L, li, fz2);
+                float *L = clt_lds + slot * SLOT_STRIDE;
+                corr_finish(c0, c1, c2, c3, L, L, li, fz2);
                 wave_lds_sync();
                 if (fr.corrs) {
                     for (int u = 0; u < 8; u++) {
                         const int r = __shfl(row, u * 8);
                         if (r < 0) continue;
-                        const float *src = clt_lds + (wave * 8 + u) * 288;
+                        const float *src = clt_lds + (wave * 8 + u) * SLOT_STRIDE;
                         float *dst = fr.corrs + (size_t)r * a.corr_stride;
 #pragma unroll
                         for (int t = 0; t < 4; t++)
@@ -292,7 +329,7 @@ void k_fused(FusedArgs a) {
                         int sy, sx;
                         if (((a.pair_mask >> pr) & 1) && ((tk->task >> (TP_TASK_CORR_BITS + pr)) & 1) &&
                             pair_src(pr, r, cy, cx, sy, sx))
-                            sum += clt_lds[(tl * 6 + pr) * 288 + (sy + 7) * 15 + sx + 7];
+                            sum += clt_lds[pair_slot(tl, pr) * SLOT_STRIDE + (sy + 7) * 15 + sx + 7];
                     }
                     store1(fr.combined + (size_t)task_num * a.combined_stride + oo, sum);
                 }
