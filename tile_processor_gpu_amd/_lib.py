@@ -8,7 +8,7 @@ import os
 
 from .geometry import CGeometry
 
-# TP_LIB_PATH selects another build of the same library (tuning variants)
+# TP_LIB_PATH selects another build of the same library (for A/B timing)
 LIB_PATH = os.environ.get("TP_LIB_PATH") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                           "libtp.so")
 

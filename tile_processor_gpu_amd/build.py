@@ -41,8 +41,8 @@ def _compile(src, out, defines, verbose):
 def build(verbose=True, out=None, defines=(), probe=True):
     """Compile csrc/tp_api.hip (the whole library is one translation unit)
     and, with probe=True, the HBM bandwidth probe csrc/hbm_probe.hip that the
-    benchmark uses (libtp_probe.so).  defines: extra -D tuning knobs (see
-    csrc/tp_common.hpp); out: library path (default: the in-tree libtp.so)."""
+    benchmark uses (libtp_probe.so).  defines: extra -D definitions for the
+    compiler; out: library path (default: the in-tree libtp.so)."""
     out = _compile(os.path.join(CSRC, "tp_api.hip"), out or LIB, defines, verbose)
     if probe:
         _compile(os.path.join(CSRC, "hbm_probe.hip"), PROBE_LIB, (), verbose)
