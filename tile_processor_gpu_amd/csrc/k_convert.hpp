@@ -581,10 +581,18 @@ __device__ __align__(256) float g_convert_sink[256];
 
 __device__ __forceinline__ void store_block(const float *slice, float *dst, int li, bool st) {
     float *base = st ? dst : g_convert_sink;
+    // store j writes unit j's whole 1 KB block: lane l copies its float4 l
+    // (the block's address from unit j's first lane)
+    const int lane = __lane_id();
+    const float *s0 = slice - (lane >> 3) * kSlice;  // unit 0's slice
+    const uint64_t b64 = (uint64_t)(uintptr_t)base;
+    const uint32_t blo = (uint32_t)b64, bhi = (uint32_t)(b64 >> 32);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        const f32x4 v = *(const f32x4 *)(slice + (j * 8 + li) * 4);
-        store4(base + (j * 8 + li) * 4, v[0], v[1], v[2], v[3]);
+        const uint64_t bj = ((uint64_t)__builtin_amdgcn_readlane(bhi, 8 * j) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(blo, 8 * j);
+        const f32x4 v = *(const f32x4 *)(s0 + j * kSlice + lane * 4);
+        store4((float *)(uintptr_t)bj + lane * 4, v[0], v[1], v[2], v[3]);
     }
     wave_lds_sync();  // the slice is the next unit's staging
 }
