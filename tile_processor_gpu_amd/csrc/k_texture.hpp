@@ -98,14 +98,17 @@ struct TexPix {
     int r, c;
     bool edge;
     float wsq, wnd2_inv;
+    float sqw[3];  // sqrt(w[color]) (the colour weights are not negative)
     // plane offset of the 3 x 3 neighbourhood's corner: pixel t + 16 dr + dc
     // is nb0 + 16 (dr + 1) + dc + 1.  Edge pixels use t itself (their
     // neighbour reads stay inside TexShared and are not used).
     int nb0;
 };
 
-__device__ __forceinline__ TexPix tex_pix(int t) {
+__device__ __forceinline__ TexPix tex_pix(int t, const float *w) {
     TexPix q;
+#pragma unroll
+    for (int c = 0; c < 3; c++) q.sqw[c] = sqrtf(fmaxf(w[c], 0.f));
     q.r = t >> 4;
     q.c = t & 15;
     auto sym = [](int i) { return i < 8 ? i : 15 - i; };
@@ -131,8 +134,13 @@ __device__ __forceinline__ void tex_constants(const TexArgs &a, float *cst) {
 }
 
 // Phases 2 and 3 of list entry e (code, tile) for thread t = pixel; U: the
-// entry's 4 x colors un-windowed planes in LDS.
-template <int COLORS>
+// entry's 4 x colors un-windowed planes in LDS.  KEEPW: the per-port
+// weights and colour spreads are output (keep_weights).  Without them the
+// debayered values only enter weighted colour distances
+// sum_col w[col] (x - y)^2, so they are scaled by sqrt(w[col]) once (the
+// colour weights are not negative) and the distances are plain sums of
+// squares.
+template <int COLORS, bool KEEPW>
 __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code, int tile,
                                               const float *U, const float *cst, const TexPix &px) {
     constexpr int colors = COLORS;
@@ -165,14 +173,14 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
             }
             d = px.edge ? v[4] : d;
             if (a.scale_shot > 0.0f) d = a.scale_shot * (d > a.min_shot ? __builtin_amdgcn_sqrtf(d) : k * d);
-            D[s / colors][col] = d * wsq;
+            D[s / colors][col] = d * (KEEPW ? wsq : px.sqw[col]);
             Uv[s / colors][col] = v[4];
         }
     } else {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             Uv[s][0] = U[s * TEX_MS + t];
-            D[s][0] = Uv[s][0] * wsq;
+            D[s][0] = Uv[s][0] * (KEEPW ? wsq : px.sqw[0]);
         }
     }
 
@@ -181,7 +189,7 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
 #pragma unroll
     for (int i = 0; i < 12; i++) out[i] = 0.f;
     float *rgba = out, *pw = out + colors + 1, *crms = out + colors + 5;
-    if (a.keep_weights) {
+    if (KEEPW) {
         float sw = 0.f;
         for (int col = 0; col < colors; col++) {
             // Spread of the 4 cameras around their mean, two-pass: the same
@@ -201,11 +209,17 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
         crms[colors] = sqrtf(sw);
     }
     // The colour distances below are sum_col w[col] ((x - y) / window^2)^2;
-    // D is divided by window^2 once.
+    // with KEEPW, D is divided by window^2 once (else it holds
+    // sqrt(w[col]) x already).
+    if (KEEPW) {
 #pragma unroll
-    for (int col = 0; col < 3; col++)
+        for (int col = 0; col < 3; col++)
 #pragma unroll
-        for (int cam = 0; cam < 4; cam++) D[cam][col] *= px.wnd2_inv;
+            for (int cam = 0; cam < 4; cam++) D[cam][col] *= px.wnd2_inv;
+    }
+    auto wdist = [&](float dc, float acc, int col) {
+        return KEEPW ? fmaf(a.w[col] * dc, dc, acc) : fmaf(dc, dc, acc);
+    };
     float p[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int i = 0, k = 0; i < 4; i++)
@@ -214,8 +228,7 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
             float d = 0.f;
 #pragma unroll
             for (int col = 0; col < colors; col++) {
-                const float dc = D[i][col] - D[j][col];
-                d = fmaf(a.w[col] * dc, dc, d);
+                d = wdist(D[i][col] - D[j][col], d, col);
             }
             d = __builtin_amdgcn_exp2f(cst[TC_PAIR + k] * d) + (float)TP_FAT_ZERO_WEIGHT;
             p[i] += d;
@@ -250,8 +263,7 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
         float d2 = 0.f;
 #pragma unroll
         for (int col = 0; col < colors; col++) {
-            const float dc = D[cam][col] - blend[col];
-            d2 = fmaf(a.w[col] * dc, dc, d2);
+            d2 = wdist(D[cam][col] - blend[col], d2, col);
         }
         p[cam] = __builtin_amdgcn_exp2f(cst[TC_KSIG] * d2) + (float)TP_FAT_ZERO_WEIGHT;
     }
@@ -281,7 +293,7 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     for (int cam = 0; cam < 4; cam++) pw[cam] = p[cam];
     const int nsl_all = 2 * colors + 6;
     if (a.tiles && a.tex_stride) {
-        const int nsl = a.keep_weights ? nsl_all : colors + 1;
+        const int nsl = KEEPW ? nsl_all : colors + 1;
         float *dst = a.tiles + (size_t)e * a.tex_stride + t;
         for (int s = 0; s < nsl; s++) dst[s * 256] = out[s];
     }
@@ -307,7 +319,7 @@ __device__ __forceinline__ int entry_pos(const TexArgs &a, int e) {
 // phase 1 runs the units of all its tiles at once (8 lanes each), then the
 // threads take the tiles' pixels one tile after the other.  Entries without
 // a texture bit, off the grid or past n are skipped (uniformly).
-template <int COLORS, int TPW>
+template <int COLORS, int TPW, bool KEEPW>
 __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared<TPW> &sh,
                                               const TexPix &px) {
     constexpr int colors = COLORS, UPT = 4 * COLORS;  // units per tile
@@ -352,12 +364,12 @@ __device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, T
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < TPW; k++)
-        if (ok[k]) texture_pixel<COLORS>(a, e0 + k, code[k], tile[k], sh.U + k * UPT * TEX_MS, sh.cst, px);
+        if (ok[k]) texture_pixel<COLORS, KEEPW>(a, e0 + k, code[k], tile[k], sh.U + k * UPT * TEX_MS, sh.cst, px);
 }
 
 // A fixed grid of workgroups loops over the list (its length may only be
 // known on the device), TPW entries at a time.
-template <int COLORS, int TPW>
+template <int COLORS, int TPW, bool KEEPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
 void k_textures_accumulate(TexArgs a) {
     __shared__ __align__(16) TexShared<TPW> sh;
@@ -367,11 +379,11 @@ void k_textures_accumulate(TexArgs a) {
         const int n_in = min(a.num_ptr[0], a.quarter);
         n = n_in + min(a.num_ptr[1], a.quarter - n_in);
     }
-    const TexPix px = tex_pix(threadIdx.x);
+    const TexPix px = tex_pix(threadIdx.x, a.w);
     if (threadIdx.x == 0) tex_constants(a, sh.cst);
     __syncthreads();
     for (int e = blockIdx.x * TPW; e < n; e += gridDim.x * TPW) {
-        texture_tiles<COLORS, TPW>(a, e, n, sh, px);
+        texture_tiles<COLORS, TPW, KEEPW>(a, e, n, sh, px);
         __syncthreads();  // the LDS of these entries is rewritten by the next ones
     }
 }

@@ -338,10 +338,14 @@ int launch_textures(const tp::TexArgs &a, int n, hipStream_t st) {
     // workgroups loop over the list, TP_TEX_WG_PER_CU per CU
     const int cus = num_cus();
     const int grid = std::min((n + TP_TEX_TPW - 1) / TP_TEX_TPW, (cus > 0 ? cus : 256) * TP_TEX_WG_PER_CU);
-    if (a.colors == 3)
-        hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
+    if (a.colors == 3 && a.keep_weights)
+        hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (a.colors == 3)
+        hipLaunchKernelGGL((tp::k_textures_accumulate<3, TP_TEX_TPW, false>), dim3(grid), dim3(256), 0, st, a);
+    else if (a.keep_weights)
+        hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW, true>), dim3(grid), dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW>), dim3(grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((tp::k_textures_accumulate<1, TP_TEX_TPW, false>), dim3(grid), dim3(256), 0, st, a);
     return check_launch("textures_accumulate");
 }
 
