@@ -38,14 +38,33 @@ def _compile(src, out, defines, verbose):
     return out
 
 
+HOST = os.path.join(HERE, "host")
+HARNESS = os.path.join(HOST, "tp_harness")
+
+
+def build_harness(verbose=True):
+    """The C++ host harness (host/tp_harness.cpp, host code only) linked
+    against the in-tree libtp.so, which it finds next to its own folder."""
+    tmp = HARNESS + ".tmp"
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HOST, "tp_harness.cpp"), "-L", HERE, "-ltp", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, HARNESS)
+    return HARNESS
+
+
 def build(verbose=True, out=None, defines=(), probe=True):
     """Compile csrc/tp_api.hip (the whole library is one translation unit)
     and, with probe=True, the HBM bandwidth probe csrc/hbm_probe.hip that the
-    benchmark uses (libtp_probe.so).  defines: extra -D definitions for the
-    compiler; out: library path (default: the in-tree libtp.so)."""
+    benchmark uses (libtp_probe.so) and the C++ host harness.  defines: extra
+    -D definitions for the compiler; out: library path (default: the in-tree
+    libtp.so)."""
     out = _compile(os.path.join(CSRC, "tp_api.hip"), out or LIB, defines, verbose)
     if probe:
         _compile(os.path.join(CSRC, "hbm_probe.hip"), PROBE_LIB, (), verbose)
+        build_harness(verbose)
     return out
 
 
