@@ -1,8 +1,8 @@
 // tp_harness.cpp - the reference's C++ test harness (src/dtt8x8.cu main,
 // :215-1050) on libtp.so: reads the harness's input files, builds the task,
 // correlation and texture lists on the host, runs the path stage by stage
-// with an untimed first iteration and `iterations` timed ones (HIP events on
-// the frame's stream), and writes the harness's result files.
+// with `warmup` untimed runs (the reference: one) and `iterations` timed ones
+// (HIP events on the frame's stream), and writes the harness's result files.
 //
 // Inputs (prefix P, as P = .../clt/main in the reference):
 //   P_chn{c}_transposed.kernel          [KV][KH][3][4][8][8] floats
@@ -33,6 +33,7 @@ struct Options {
     std::string prefix, out = ".";
     tp_geometry g = {2592, 1936, 164, 123};
     int iterations = 10;
+    int warmup = 1;
     int radius = 7;
     float fat_zero = 30.0f;  // the harness's value for raw pixel data (src/dtt8x8.cu:725)
     int colors = 3;
@@ -43,7 +44,8 @@ struct Options {
 void usage(const char *prog) {
     std::printf(
         "usage: %s --prefix P [--out DIR] [--width W --height H --kernels-hor KH --kernels-vert KV]\n"
-        "          [--iterations N] [--radius R] [--fat-zero F] [--colors 3|1] [--keep-weights 0|1]\n"
+        "          [--iterations N] [--warmup N] [--radius R] [--fat-zero F] [--colors 3|1]\n"
+        "          [--keep-weights 0|1]\n"
         "          [--save-clt] [--imclt-all] [--lists-only]\n"
         "Runs convert_correct_tiles, imclt_rbg (48 launches; --imclt-all: one), correlate2D,\n"
         "textures_accumulate, prepare_texture_list and generate_RBGA of libtp.so on the\n"
@@ -66,6 +68,7 @@ bool parse(int argc, char **argv, Options &o) {
         else if (a == "--kernels-hor") o.g.kernels_hor = std::atoi(next("--kernels-hor"));
         else if (a == "--kernels-vert") o.g.kernels_vert = std::atoi(next("--kernels-vert"));
         else if (a == "--iterations") o.iterations = std::atoi(next("--iterations"));
+        else if (a == "--warmup") o.warmup = std::atoi(next("--warmup"));
         else if (a == "--radius") o.radius = std::atoi(next("--radius"));
         else if (a == "--fat-zero") o.fat_zero = (float)std::atof(next("--fat-zero"));
         else if (a == "--colors") o.colors = std::atoi(next("--colors"));
@@ -77,6 +80,7 @@ bool parse(int argc, char **argv, Options &o) {
     }
     if (o.prefix.empty()) throw tph::Error("--prefix is required");
     if (o.iterations < 1) throw tph::Error("--iterations must be positive");
+    if (o.warmup < 0) throw tph::Error("--warmup must not be negative");
     return true;
 }
 
@@ -84,10 +88,10 @@ std::string chn(const std::string &p, int c, const char *suffix) {
     return p + "_chn" + std::to_string(c) + suffix;
 }
 
-// One stage: an untimed first run, then `n` timed runs bracketed by events.
+// One stage: `warm` untimed runs, then `n` timed runs bracketed by events.
 template <class F>
-float timed(const char *name, int n, hipStream_t st, F &&run) {
-    run();
+float timed(const char *name, int warm, int n, hipStream_t st, F &&run) {
+    for (int i = 0; i < warm; i++) run();
     tph::hip_check(hipStreamSynchronize(st), name);
     hipEvent_t a, b;
     tph::hip_check(hipEventCreate(&a), "hipEventCreate");
@@ -141,15 +145,15 @@ int run(const Options &o) {
         std::printf("%zu tiles, %zu correlations, %zu texture tiles\n", tasks.size(), fr.num_corr(),
                     fr.num_texture_list());
 
-        timed("convert_correct_tiles", o.iterations, st, [&] { fr.convert(); });
+        timed("convert_correct_tiles", o.warmup, o.iterations, st, [&] { fr.convert(); });
         fr.sync();
         if (o.save_clt)
             for (int c = 0; c < TP_NUM_CAMS; c++) tph::write_floats(o.out + "/main_chn" + std::to_string(c) + ".clt", fr.clt(c));
 
         if (o.all_imclt)
-            timed("imclt_rbg_all", o.iterations, st, [&] { fr.imclt_all(); });
+            timed("imclt_rbg_all", o.warmup, o.iterations, st, [&] { fr.imclt_all(); });
         else
-            timed("imclt_rbg (48 launches)", o.iterations, st, [&] { fr.imclt_passes(); });
+            timed("imclt_rbg (48 launches)", o.warmup, o.iterations, st, [&] { fr.imclt_passes(); });
         // the result files come from one more run on cleared planes (repeated
         // 48-launch runs add up in the margin rows and columns)
         fr.clear_rbg();
@@ -158,7 +162,7 @@ int run(const Options &o) {
         fr.sync();
         for (int c = 0; c < TP_NUM_CAMS; c++) tph::write_floats(o.out + "/main_chn" + std::to_string(c) + ".rbg", fr.rbg(c));
 
-        timed("correlate2D", o.iterations, st, [&] { fr.correlate(3, 0.25f, 0.25f, 0.5f, o.fat_zero); });
+        timed("correlate2D", o.warmup, o.iterations, st, [&] { fr.correlate(3, 0.25f, 0.25f, 0.5f, o.fat_zero); });
         fr.sync();
         tph::write_floats(o.out + "/main_corr.corr", fr.corrs());
 
@@ -166,11 +170,11 @@ int run(const Options &o) {
         tp.colors = o.colors;
         tp.is_lwir = o.colors == 1;
         tp.keep_weights = o.keep_weights;
-        timed("textures_accumulate", o.iterations, st, [&] { fr.textures(tp); });
+        timed("textures_accumulate", o.warmup, o.iterations, st, [&] { fr.textures(tp); });
         fr.sync();
         tph::write_floats(o.out + "/texture.rgba", fr.texture_tiles());
 
-        timed("prepare_texture_list", o.iterations, st, [&] { fr.prepare_texture_list(); });
+        timed("prepare_texture_list", o.warmup, o.iterations, st, [&] { fr.prepare_texture_list(); });
         fr.sync();
         std::array<int32_t, 4> woi = fr.woi();
         std::array<int32_t, 8> n = fr.num_texture_tiles();
@@ -178,7 +182,7 @@ int run(const Options &o) {
         std::printf("num_texture_tiles=(%d(%d), %d(%d), %d(%d), %d(%d))\n", n[0], n[1], n[2], n[3], n[4], n[5],
                     n[6], n[7]);
 
-        timed("generate_RBGA", o.iterations, st, [&] { fr.generate_rbga(tp); });
+        timed("generate_RBGA", o.warmup, o.iterations, st, [&] { fr.generate_rbga(tp); });
         fr.sync();
         woi = fr.woi();
         std::printf("WOI x=%d, y=%d, width=%d, height=%d\n", woi[0], woi[1], woi[2], woi[3]);
