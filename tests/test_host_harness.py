@@ -104,7 +104,7 @@ def test_harness_results_match_oracle_and_binding(files, orc):
     out.mkdir(exist_ok=True)
     print(f"[gpu harness] {g}", flush=True)
     r = _run(["--prefix", prefix, "--out", out, "--iterations", 2, "--save-clt", "--fat-zero", FZ,
-              "--radius", 7, "--keep-weights", 1] + _geom_args(g))
+              "--radius", 7, "--keep-weights", 1, "--fused"] + _geom_args(g))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Average correlate2D run time" in r.stdout
 
@@ -117,6 +117,8 @@ def test_harness_results_match_oracle_and_binding(files, orc):
     ref = orc.correlate2D(g, clt64, 3, (0.25, 0.25, 0.5), FZ, idx, 7)
     got = kernelgen.read_floats(str(out / "main_corr.corr"), (len(idx), 225))
     assert rel_err(got, ref) < TOL
+    fused = kernelgen.read_floats(str(out / "main_corr_fused.corr"), (len(idx), 225))
+    assert rel_err(fused, got) <= 1e-6 and rel_err(fused, ref) < TOL
     planes = orc.imclt_frame(g, clt64)
     for c in range(4):
         got = kernelgen.read_floats(str(out / f"main_chn{c}.rbg"), (3, g.img_height + 8, g.img_width + 8))

@@ -13,6 +13,8 @@
 //   main_chn{c}.clt          with --save-clt: the forward CLT
 //   main_chn{c}.rbg          imclt planes, 3 x (H + 8) rows of W + 8
 //   main_corr.corr           (2r+1)^2 floats per correlation-list entry
+//   main_corr_fused.corr     with --fused: the same from tp_convert_correlate
+//                            (rows 6 tile + pair: the list's order here)
 //   texture.rgba             textures_accumulate tiles, one per list entry
 //   texture_rgba.rgba        generate_RBGA image, (tilesx + 1) 8 columns,
 //                            (tilesy + 1) 8 rows per slice, colors + 1 slices
@@ -38,7 +40,7 @@ struct Options {
     float fat_zero = 30.0f;  // the harness's value for raw pixel data (src/dtt8x8.cu:725)
     int colors = 3;
     int keep_weights = 1;    // the harness's textures_accumulate run (:274)
-    bool save_clt = false, lists_only = false, all_imclt = false;
+    bool save_clt = false, lists_only = false, all_imclt = false, fused = false;
 };
 
 void usage(const char *prog) {
@@ -46,7 +48,7 @@ void usage(const char *prog) {
         "usage: %s --prefix P [--out DIR] [--width W --height H --kernels-hor KH --kernels-vert KV]\n"
         "          [--iterations N] [--warmup N] [--radius R] [--fat-zero F] [--colors 3|1]\n"
         "          [--keep-weights 0|1]\n"
-        "          [--save-clt] [--imclt-all] [--lists-only]\n"
+        "          [--save-clt] [--imclt-all] [--fused] [--lists-only]\n"
         "Runs convert_correct_tiles, imclt_rbg (48 launches; --imclt-all: one), correlate2D,\n"
         "textures_accumulate, prepare_texture_list and generate_RBGA of libtp.so on the\n"
         "reference harness's input files and writes its result files.\n",
@@ -75,6 +77,7 @@ bool parse(int argc, char **argv, Options &o) {
         else if (a == "--keep-weights") o.keep_weights = std::atoi(next("--keep-weights"));
         else if (a == "--save-clt") o.save_clt = true;
         else if (a == "--imclt-all") o.all_imclt = true;
+        else if (a == "--fused") o.fused = true;
         else if (a == "--lists-only") o.lists_only = true;
         else throw tph::Error("unknown argument " + a);
     }
@@ -165,6 +168,12 @@ int run(const Options &o) {
         timed("correlate2D", o.warmup, o.iterations, st, [&] { fr.correlate(3, 0.25f, 0.25f, 0.5f, o.fat_zero); });
         fr.sync();
         tph::write_floats(o.out + "/main_corr.corr", fr.corrs());
+        if (o.fused) {
+            timed("convert_correlate (fused)", o.warmup, o.iterations, st,
+                  [&] { fr.convert_correlate(3, 0.25f, 0.25f, 0.5f, o.fat_zero); });
+            fr.sync();
+            tph::write_floats(o.out + "/main_corr_fused.corr", fr.corrs_fused());
+        }
 
         tph::TextureParams tp;
         tp.colors = o.colors;

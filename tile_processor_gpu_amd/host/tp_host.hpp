@@ -346,6 +346,22 @@ class Frame {
         check(tp_imclt_rbg_all(&g_, p_clt_.get(), p_rbg_.get(), rbg_[0].stride(), st_), "tp_imclt_rbg_all");
     }
 
+    // forward CLT and pair correlation in one pass (no counterpart in the
+    // reference): rows 6 task + pair of corrs_fused(), which for a full
+    // line-scan task list are the correlation list's order; the CLT is not
+    // stored (store_clt: also into the CLT buffers)
+    void convert_correlate(int colors, float scale0, float scale1, float scale2, float fat_zero,
+                           bool store_clt = false, const tp_fused_config *config = nullptr) {
+        const int n = 2 * radius_ + 1;
+        const size_t rows = std::max<size_t>(1, TP_NUM_PAIRS * tasks_host_.size());
+        if ((size_t)fused_.height() < rows) fused_ = PitchedArray(n * n, (int)rows);
+        check(tp_convert_correlate(&g_, p_offsets_.get(), p_kernels_.get(), p_images_.get(), tasks_.get(),
+                                   store_clt ? p_clt_.get() : nullptr, images_[0].stride(),
+                                   (int)tasks_host_.size(), 0, colors, scale0, scale1, scale2, fat_zero,
+                                   fused_.stride(), radius_, fused_.get(), nullptr, 0, 0x3F, config, st_),
+              "tp_convert_correlate");
+    }
+
     void correlate(int colors, float scale0, float scale1, float scale2, float fat_zero) {
         check(tp_correlate2D(&g_, p_clt_.get(), colors, scale0, scale1, scale2, fat_zero, num_corr_,
                              corr_idx_.get(), corrs_.stride(), radius_, corrs_.get(), st_),
@@ -391,6 +407,7 @@ class Frame {
     std::vector<float> clt(int cam) const { return clt_.at(cam).download(d_.clt_floats()); }
     std::vector<float> rbg(int cam) const { return rbg_.at(cam).download(); }
     std::vector<float> corrs() const { return corrs_.download((int)num_corr_); }
+    std::vector<float> corrs_fused() const { return fused_.download((int)(TP_NUM_PAIRS * tasks_host_.size())); }
     std::vector<float> texture_tiles() const { return tex_tiles_.download((int)num_tex_list_); }
     std::vector<float> rbga() const { return rbga_.download(); }
     std::array<int32_t, 4> woi() const {
@@ -426,7 +443,7 @@ class Frame {
     DeviceArray<tp_task> tasks_;
     std::vector<tp_task> tasks_host_;
     size_t num_corr_ = 0, num_tex_list_ = 0;
-    PitchedArray corrs_, tex_tiles_, rbga_;
+    PitchedArray corrs_, fused_, tex_tiles_, rbga_;
 };
 
 }  // namespace tph
