@@ -12,7 +12,7 @@ Prints one JSON line: frames/s, the HBM roofline of the dominant kernel
 (algorithmic bytes / its HIP-event duration), the error against the FP64
 oracle on a sample of tiles, and the CPU oracle timed on a bounded sample.
 
-  python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+  python bench.py [--gpus 1] [--steps 200] [--warmup 300] [--settle-ms 100]
 """
 import argparse
 import json
@@ -456,7 +456,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=300,
-                    help="untimed frames first: the clocks take a few dozen launches to settle")
+                    help="untimed frames before the timed ones")
+    ap.add_argument("--settle-ms", type=float, default=100.0,
+                    help="frames run for this long before the warm-up: the clocks take about 50 "
+                         "frames to settle after the host's set-up")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=242, help="tile rows for the CPU baseline (0: skip)")
     ap.add_argument("--err-tiles", type=int, default=512,
@@ -546,26 +549,41 @@ def main():
         fb.convert()
         fb.correlate(3, SCALES, FAT_ZERO)
 
+    # After the host has been idle (the frame's set-up above), the GPU needs
+    # about 50 frames (30 ms) to reach its steady clocks: the first 25 run
+    # about 5% slower.  Frames are run for settle_ms before the warm-up
+    # steps, so that a short --warmup still times the steady state.
+    t0 = time.perf_counter()
+    settle_steps = 0
+    while 1e3 * (time.perf_counter() - t0) < args.settle_ms:
+        for _ in range(8):
+            step()
+        settle_steps += 8
+        torch.cuda.synchronize()
+    settle = {"ms": round(1e3 * (time.perf_counter() - t0), 1), "steps": settle_steps}
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    log(f"warmup {args.warmup} done")
+    log(f"settle {settle}, warmup {args.warmup} done")
 
+    # Events: s[k] before step k's convert, m[k] between its convert and its
+    # correlate, s[K] after the last step (two records per step).
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    s_ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    m_ev = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(K):
-        ev[k][0].record()
+        s_ev[k].record()
         fb.convert()
-        ev[k][1].record()
+        m_ev[k].record()
         fb.correlate(3, SCALES, FAT_ZERO)
-        ev[k][2].record()
+    s_ev[K].record()
     torch.cuda.synchronize()
     t_end = time.perf_counter()
     total_s = t_end - t_start
-    conv_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-    corr_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    conv_ms = float(np.mean([s_ev[k].elapsed_time(m_ev[k]) for k in range(K)]))
+    corr_ms = float(np.mean([m_ev[k].elapsed_time(s_ev[k + 1]) for k in range(K)]))
     log(f"timed {K} steps: {1e3 * total_s / K:.3f} ms/step; convert {conv_ms:.3f} ms, "
         f"correlate {corr_ms:.3f} ms")
 
@@ -636,6 +654,7 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
+        "settle": settle,
         "ms_per_step": round(1e3 * total_s / K, 4),
         "higher_is_better": True,
         "scaling": "weak",
