@@ -110,13 +110,13 @@ __device__ __forceinline__ void imclt_unfold(const float (&t0)[8], const float (
 
 __device__ __forceinline__ void load_rows(const float *src, int li, float (&q0)[8], float (&q1)[8],
                                           float (&q2)[8], float (&q3)[8]) {
-    const float4 *p = (const float4 *)(src + li * 8);
+    gf32x4 *p = gptr4(src + li * 8);  // global loads (src is a device-memory CLT block)
     float *q[4] = {q0, q1, q2, q3};
 #pragma unroll
     for (int qq = 0; qq < 4; qq++) {
-        const float4 u = p[qq * 16], v = p[qq * 16 + 1];
-        q[qq][0] = u.x; q[qq][1] = u.y; q[qq][2] = u.z; q[qq][3] = u.w;
-        q[qq][4] = v.x; q[qq][5] = v.y; q[qq][6] = v.z; q[qq][7] = v.w;
+        const f32x4 u = p[qq * 16], v = p[qq * 16 + 1];
+        q[qq][0] = u[0]; q[qq][1] = u[1]; q[qq][2] = u[2]; q[qq][3] = u[3];
+        q[qq][4] = v[0]; q[qq][5] = v[1]; q[qq][6] = v[2]; q[qq][7] = v[3];
     }
 }
 
@@ -210,7 +210,7 @@ void k_imclt_frame(Geo g, const float *const *clt, float *const *rbg,
     float *plane = rbg[cam] + dstride * (size_t)PH * color;
     for (int i = threadIdx.x; i < 64 * 64; i += 256) {
         const int y = Y0 + (i >> 6), x = X0 + (i & 63);
-        if (y < PH && x < PW) plane[(size_t)y * dstride + x] = acc[(i >> 6) * AP + (i & 63)];
+        if (y < PH && x < PW) store1(plane + (size_t)y * dstride + x, acc[(i >> 6) * AP + (i & 63)]);
     }
 }
 
