@@ -320,11 +320,12 @@ int tp_imclt_rbg_all(const tp_geometry *geom, float **gpu_clt, float **gpu_rbg, 
         return fail(TP_EINVAL, "tp_imclt_rbg_all: dstride < image width + 8%s", "");
     g_err.clear();
     tp::Geo g = tp::make_geo(*geom);
-    // output-stationary: one workgroup per 64x64 block of one plane
-    const int bx = (g.W + 8 + 63) / 64, by = (g.H + 8 + 63) / 64;
-    dim3 grid(bx * by, TP_NUM_COLORS, TP_NUM_CAMS);
-    hipLaunchKernelGGL(tp::k_imclt_frame, grid, dim3(256), 0, as_stream(stream), g,
-                       (const float *const *)gpu_clt, gpu_rbg, dstride);
+    // strips of 15 column groups x bands of IMS_BAND row groups, each plane
+    constexpr int band = tp::IMS_BAND;
+    dim3 grid((g.TX + 1 + tp::IMS_OWN - 1) / tp::IMS_OWN, (g.TY + 1 + band - 1) / band,
+              TP_NUM_COLORS * TP_NUM_CAMS);
+    hipLaunchKernelGGL(tp::k_imclt_strip, grid, dim3(256), 0, as_stream(stream), g,
+                       (const float *const *)gpu_clt, gpu_rbg, dstride, band);
     return check_launch("imclt_rbg_all");
 }
 

@@ -22,7 +22,6 @@
 #define TP_FUSED_QWAVES 3      // fused kernel, FP32 form: minimum waves per SIMD
 #define TP_FUSED_WG_WAVES 4    // fused kernel: wavefronts per workgroup
 #define TP_STRIP_ROWS 2        // fused kernel: tile rows per strip of the visiting order
-#define TP_IMCLT_WAVES 5       // whole-frame inverse CLT: minimum waves per SIMD
 #define TP_TEX_WG_PER_CU 5     // textures: workgroups per CU in the (list-looping) grid
 #define TP_TEX_TPW 2           // textures: list entries (tiles) per workgroup at a time
 #define TP_TEX_WAVES 4         // textures: minimum waves per SIMD
