@@ -187,7 +187,10 @@ int tp_imclt_rbg(const tp_geometry *geom,
                  size_t dstride,
                  tp_stream_t stream);
 
-/* Whole-frame variant: for each camera, all colors, all 4 passes in order.
+/* Whole-frame variant: for each camera and color, the plane that the 4
+ * passes of tp_imclt_rbg give (each pixel the sum of the 4 windowed tile
+ * outputs that cover it, rounded in another order), in one launch.  Every
+ * pixel of the (W + 8) x (H + 8) planes is written (nothing accumulates).
  * gpu_rbg: [cam] -> planes as in tp_imclt_rbg (device pointer array). */
 int tp_imclt_rbg_all(const tp_geometry *geom,
                      float **gpu_clt,
