@@ -549,23 +549,37 @@ __device__ __forceinline__ void nn_conv_2blocks(const uint16_t *in, const u32x4 
     }
 }
 
-// bias + ReLU, BF16, channel-last store of one 32-slot block at
-// out[idx * NN_CS] (the caller sends slots past the tile to a junk
-// position).  Lane i holds channels 8 j + 4 (i / 32) .. + 3 (j = 0..3) of
+// The layer's biases in the accumulator layout (lane i: channels
+// 8 j + 4 (i / 32) .. + 3 at elements 4 j .. 4 j + 3): the first product of
+// a block's chain adds them, so the epilogue does not.
+__device__ __forceinline__ f32x16 nn_bias_acc(const float *bias, int lane) {
+    const int half = lane >> 5;
+    f32x16 v;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float4 b = *(const float4 *)(bias + 8 * j + 4 * half);
+        v[4 * j] = b.x;
+        v[4 * j + 1] = b.y;
+        v[4 * j + 2] = b.z;
+        v[4 * j + 3] = b.w;
+    }
+    return v;
+}
+
+// ReLU (the accumulator holds the bias already), BF16, channel-last store
+// of one 32-slot block at out[idx * NN_CS] (the caller sends slots past the
+// tile to a junk position).  Lane i holds channels 8 j + 4 (i / 32) .. + 3 (j = 0..3) of
 // its slot; the two wavefront halves swap half their words
 // (v_permlane32_swap), after which the lower half holds channels 0..7 and
 // 16..23 and the upper half 8..15 and 24..31, each run stored as one
-// 16-byte write.  bias: the layer's 32 biases in LDS (this lane's rows
-// 8 j + 4 (i / 32) .. + 3 are one float4 each).
-__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, const float *bias, int idx,
-                                               int lane) {
+// 16-byte write.
+__device__ __forceinline__ void nn_store_block(uint16_t *out, const f32x16 &acc, int idx, int lane) {
     const int half = lane >> 5;
     uint32_t w[8];  // w[2 j], w[2 j + 1]: channels 8 j + 4 half + (0, 1), (2, 3)
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const float4 b = *(const float4 *)(bias + 8 * j + 4 * half);
-        w[2 * j] = relu_bf16x2(bf16pair(acc[4 * j] + b.x, acc[4 * j + 1] + b.y));
-        w[2 * j + 1] = relu_bf16x2(bf16pair(acc[4 * j + 2] + b.z, acc[4 * j + 3] + b.w));
+        w[2 * j] = relu_bf16x2(bf16pair(acc[4 * j], acc[4 * j + 1]));
+        w[2 * j + 1] = relu_bf16x2(bf16pair(acc[4 * j + 2], acc[4 * j + 3]));
     }
     // swap(x, y): lower half keeps x and gets the upper half's x; upper half
     // gets the lower half's y and keeps y
@@ -612,17 +626,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, int n, float *out) {
     __shared__ __align__(16) uint16_t X0[NN_PROWS * 8];   // x, padded, channel 7 zero
     __shared__ __align__(16) uint16_t H1[NN_PROWS * NN_CS];
-    __shared__ __align__(16) uint16_t H2[256 * NN_CS];    // by slot
-    __shared__ __align__(16) float Gs[256 * NN_GS];       // by slot
+    // h2 on the padded grid as h1: the positions of the slots past the tile
+    // and the ring stay 0, so G is 0 there
+    __shared__ __align__(16) uint16_t H2[NN_PROWS * NN_CS];
+    // G by slot + NN_POFF, with NN_POFF zero rows before and after: the
+    // 9-tap shift-add reads its neighbours unmasked (outside the tile: 0)
+    __shared__ __align__(16) float Gs[(256 + 2 * NN_POFF) * NN_GS];
     __shared__ __align__(16) float bias[2][NN_C];       // b1, b2
     __shared__ float red[16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int co = lane & 31, half = lane >> 5;
     if (t < 2 * NN_C) bias[t >> 5][t & 31] = bf16f(prm[(t < NN_C ? NN_OFF_B1 : NN_OFF_B2 - NN_C) + t]);
     for (int i = t; i < NN_PROWS * 4; i += 256) ((uint32_t *)X0)[i] = 0u;
-    for (int i = t; i < 256 * NN_CS / 2; i += 256) ((uint32_t *)H2)[i] = 0u;
+    for (int i = t; i < NN_PROWS * NN_CS / 2; i += 256) ((uint32_t *)H2)[i] = 0u;
     for (int i = t; i < NN_PROWS * NN_CS / 2; i += 256) ((uint32_t *)H1)[i] = 0u;
-    for (int i = t; i < 256 * NN_GS; i += 256) Gs[i] = 0.f;  // finite everywhere (read under a select)
+    for (int i = t; i < (256 + 2 * NN_POFF) * NN_GS; i += 256) Gs[i] = 0.f;
     u32x4 A1[5], A2[18], B3[2];
 #pragma unroll
     for (int kc = 0; kc < 5; kc++) A1[kc] = nn_wfrag<NN_IN>(prm + NN_OFF_W1, co, 0, 2 * kc + half, NN_IN);
@@ -646,9 +664,8 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
     const f32x16 zero = {};
     const int s0 = 64 * wave + co, s1 = s0 + 32;  // this lane's slot column in the wave's 2 blocks
     // thread t = slot t in the head
-    const int sy = t >> 4, sx = t & 15;
+    const int sx = t & 15;
     const bool px = nn_slot_in_tile(t);
-    const unsigned nbm = px ? nbr_mask_yx(sy, sx) : 0u;
     // the head of the previous tile: one thread combines the wavefronts'
     // partial sums in red (after the second barrier of the next tile, which
     // every wavefront reaches only after writing them; red is rewritten only
@@ -683,21 +700,23 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
         __syncthreads();
         load_x(tile + gridDim.x);
         {
-            f32x16 a0 = zero, a1 = zero;
+            const f32x16 b1 = nn_bias_acc(bias[0], lane);
+            f32x16 a0 = b1, a1 = b1;
             nn_conv_2blocks<8, 5>(X0, A1, s0, s1, lane, a0, a1);
-            nn_store_block(H1, a0, bias[0], nn_slot_in_tile(s0) ? s0 + NN_POFF : NN_PJUNK, lane);
-            nn_store_block(H1, a1, bias[0], nn_slot_in_tile(s1) ? s1 + NN_POFF : NN_PJUNK, lane);
+            nn_store_block(H1, a0, nn_slot_in_tile(s0) ? s0 + NN_POFF : NN_PJUNK, lane);
+            nn_store_block(H1, a1, nn_slot_in_tile(s1) ? s1 + NN_POFF : NN_PJUNK, lane);
         }
         __syncthreads();
         // the head of the previous tile, by the lane of the wavefront with
         // the fewest pixels, off the path to the next barrier's other waits
         if (t == 192 && prev >= 0) finish(prev);
         {
-            f32x16 a0 = zero, a1 = zero;
+            const f32x16 b2 = nn_bias_acc(bias[1], lane);
+            f32x16 a0 = b2, a1 = b2;
             nn_conv_2blocks<NN_C, 18>(H1, A2, s0, s1, lane, a0, a1);
             // h2 of the wave's own 64 slots
-            nn_store_block(H2, a0, bias[1], s0, lane);
-            nn_store_block(H2, a1, bias[1], s1, lane);
+            nn_store_block(H2, a0, nn_slot_in_tile(s0) ? s0 + NN_POFF : NN_PJUNK, lane);
+            nn_store_block(H2, a1, nn_slot_in_tile(s1) ? s1 + NN_POFF : NN_PJUNK, lane);
         }
         wave_lds_sync();
         {
@@ -705,8 +724,9 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
             f32x16 g0 = zero, g1 = zero;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const u32x4 a0 = *(const u32x4 *)(H2 + (64 * wave + co) * NN_CS + 16 * s + 8 * half);
-                const u32x4 a1 = *(const u32x4 *)(H2 + (64 * wave + 32 + co) * NN_CS + 16 * s + 8 * half);
+                const u32x4 a0 = *(const u32x4 *)(H2 + (64 * wave + co + NN_POFF) * NN_CS + 16 * s + 8 * half);
+                const u32x4 a1 =
+                    *(const u32x4 *)(H2 + (64 * wave + 32 + co + NN_POFF) * NN_CS + 16 * s + 8 * half);
                 g0 = nn_mfma(a0, B3[s], g0);
                 g1 = nn_mfma(a1, B3[s], g1);
             }
@@ -715,20 +735,24 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
                     const int q0 = 64 * wave + 8 * (v >> 2) + 4 * half + (v & 3);  // slots past the tile: junk
-                    Gs[q0 * NN_GS + co] = g0[v];
-                    Gs[(q0 + 32) * NN_GS + co] = g1[v];
+                    Gs[(q0 + NN_POFF) * NN_GS + co] = g0[v];
+                    Gs[(q0 + 32 + NN_POFF) * NN_GS + co] = g1[v];
                 }
             }
         }
         __syncthreads();
-        // z = b3 + the 9-tap shift-add of G (neighbours outside the tile: 0)
+        // z = b3 + the 9-tap shift-add of G: a neighbour outside the tile is
+        // a slot past it (G = 0, its h2 row is the zero ring) or a zero row
+        // of Gs, so every tap adds as read (what a masked tap adds: 0)
         float z0 = b30, z1 = b31;
+        {
+            const float *gt = Gs + (t + NN_POFF) * NN_GS;
 #pragma unroll
-        for (int r = 0; r < 9; r++) {
-            const bool in = (nbm >> r) & 1u;
-            const float *gq = Gs + min(255, max(0, t + NN_PW * (r / 3 - 1) + r % 3 - 1)) * NN_GS;
-            z0 += in ? gq[r] : 0.f;
-            z1 += in ? gq[9 + r] : 0.f;
+            for (int r = 0; r < 9; r++) {
+                const float *gq = gt + (NN_PW * (r / 3 - 1) + r % 3 - 1) * NN_GS;
+                z0 += gq[r];
+                z1 += gq[9 + r];
+            }
         }
         // softmax statistics of the head (as head_stats) with one barrier:
         // each wavefront reduces relative to its own maximum, thread 0
