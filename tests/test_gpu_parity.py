@@ -216,6 +216,28 @@ def test_imclt_frame_matches_oracle(small, orc):
             assert rel_err(got[color], ref[cam][color]) < TOL
 
 
+def test_imclt_frame_other_grid(orc):
+    """Whole-frame inverse CLT on a 34 x 26 tile grid: 35 x 27 groups of 8
+    output columns / rows, so the last strip of 15 column groups and the
+    last band of row groups are partial.  Every plane pixel is written
+    (NaN before) and matches the oracle."""
+    g = Geometry(272, 208)
+    cfg = synth.config_small(seed=6, geom=g)
+    print(f"[gpu parity] imclt odd grid {g}", flush=True)
+    clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], cfg["tasks"])
+    dstride = g.img_width + 8 + 3
+    planes = [torch.full((3, g.img_height + 8, dstride), float("nan"), device="cuda:0") for _ in range(4)]
+    clts = [torch.from_numpy(c.astype(np.float32)).cuda() for c in clt64]
+    tp.imclt_rbg_all(g, tp.ptr_array(clts), tp.ptr_array(planes), dstride)
+    torch.cuda.synchronize()
+    ref = orc.imclt_frame(g, clt64, dstride)
+    for cam in range(4):
+        got = planes[cam].cpu().numpy()[:, :, :g.img_width + 8]
+        assert np.isfinite(got).all(), cam
+        for color in range(3):
+            assert rel_err(got[color], ref[cam][color][:, :g.img_width + 8]) < TOL, (cam, color)
+
+
 def test_full_frame_tdac_and_correlation_peak(orc):
     """Full 2592 x 1936 frame, size-independent properties: identity kernels
     on the 8-px grid reconstruct every Bayer plane (TDAC) and an all-zero
