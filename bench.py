@@ -354,9 +354,9 @@ def measure_stages(cfg, fb, idx, reps):
     out["centroid_ms"] = round(median_ms(lambda: tp.corr_disparity(comb, 225, g.num_tiles, RADIUS, 1,
                                                                     disp), reps), 4)
     # inverse CLT of the whole frame (4 cameras x 3 colors, all four tile
-    # parities) into (W + 8) x (H + 8) planes: one output-stationary launch,
-    # and the reference's 48 per-pass launches (read-modify-write) of the
-    # same planes
+    # parities) into (W + 8) x (H + 8) planes: one launch (column strips,
+    # every pixel written once), and the reference's 48 per-pass launches
+    # (read-modify-write) of the same planes
     dstride = g.img_width + 8
     planes = [torch.zeros(3 * (g.img_height + 8) * dstride, device=dev) for _ in range(4)]
     p_planes = tp.ptr_array(planes)
