@@ -414,6 +414,11 @@ def measure_stages(cfg, fb, idx, reps):
         flops = NN_FLOP_PER_TILE * g.num_tiles
         out["nn_infer_TFLOPs"] = round(flops / (ms_nn * 1e-3) / 1e12, 1)
         out["nn_infer_frac_of_bf16_peak"] = round(flops / (ms_nn * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)
+        # the same network reading the correlation rows directly (no
+        # [tiles][7][225] input buffer): the input assembly and inference of
+        # a frame in one launch
+        out["nn_from_corrs_ms"] = round(median_ms(
+            lambda: model.infer_corrs(g, fb.tasks, g.num_tiles, fb.corrs, fb.corr_stride), reps, warmup=20), 4)
         out["nn_frame_ms_fp32"] = round(
             median_ms(lambda: model.forward(x, g.num_tiles), 3, warmup=1), 4)
         del x

@@ -524,6 +524,13 @@ int tp_nn_forward(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, fl
 int tp_nn_infer(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, float *gpu_out,
                 tp_stream_t stream);
 
+/* tp_nn_input followed by tp_nn_infer in one pass: the network's input is
+ * assembled from the correlation rows (6 task + pair, radius 7) as it is
+ * read, without the [num_tasks][7][225] buffer; the same results. */
+int tp_nn_infer_corrs(const tp_geometry *geom, const uint16_t *gpu_params_bf16, const tp_task *gpu_tasks,
+                      int num_tasks, const float *gpu_corrs, size_t corr_stride, float *gpu_out,
+                      tp_stream_t stream);
+
 /* Adds the gradient of the mean training loss over the tiles with a finite
  * target (huber(d - target) + conf_weight * BCE(confidence, |d - target| <
  * 0.5)) to gpu_grads[TP_NN_PARAMS] and the loss to gpu_loss[0];

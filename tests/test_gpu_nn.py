@@ -49,6 +49,36 @@ def test_nn_input_matches_oracle(orc):
     assert rel_err(xs[:, 0], ref) <= 1e-6
 
 
+def test_nn_infer_corrs_equals_input_then_infer():
+    """tp_nn_infer_corrs (input assembled while reading the correlation rows)
+    gives exactly tp_nn_input + tp_nn_infer, also for tasks with pairs off,
+    a task off the grid and a list shorter than the frame."""
+    import ctypes
+    g, fb, _ = _inputs(seed=8)
+    model = nn.DisparityCNN(seed=11)
+    tasks = fb.tasks[:40 * g.num_tiles].clone()
+    tv = tasks.view(-1, 40)
+    rng = np.random.default_rng(2)
+    words = tv[:, :4].contiguous().view(torch.int32).cpu().numpy()[:, 0].copy()
+    for i in rng.choice(g.num_tiles, 200, replace=False):
+        words[i] &= ~(int(rng.integers(1, 63)) << 4)  # some pairs off
+    tv[:, :4] = torch.from_numpy(words.astype(np.int32).view(np.uint8).reshape(-1, 4)).cuda()
+    txy = tv[:, 4:8].contiguous().view(torch.int32).cpu().numpy()[:, 0].copy()
+    txy[5] = (g.tilesy + 3) << 16  # off the grid
+    tv[:, 4:8] = torch.from_numpy(txy.astype(np.int32).view(np.uint8).reshape(-1, 4)).cuda()
+    for n in (g.num_tiles, 517):
+        x = torch.zeros((n, 7, 225), device="cuda:0")
+        _lib.call("tp_nn_input", ctypes.byref(g.c()), ctypes.c_void_p(tasks.data_ptr()), n,
+                  ctypes.c_void_p(fb.corrs.data_ptr()), fb.corr_stride, ctypes.c_void_p(x.data_ptr()),
+                  tp._stream(None))
+        two = model.infer(x, n)
+        one = model.infer_corrs(g, tasks, n, fb.corrs, fb.corr_stride)
+        torch.cuda.synchronize()
+        print(f"[gpu nn] infer_corrs n={n}", flush=True)
+        assert torch.equal(one, two), n
+        assert torch.isfinite(one).all()
+
+
 @pytest.fixture(scope="module")
 def net_case(orc):
     g, fb, x = _inputs(seed=4)

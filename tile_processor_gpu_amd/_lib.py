@@ -48,6 +48,7 @@ PROTOTYPES = {
     "tp_nn_input": [_G, _P, _I, _P, _SZ, _P, _P],
     "tp_nn_forward": [_P, _P, _I, _P, _P, _P],
     "tp_nn_infer": [_P, _P, _I, _P, _P],
+    "tp_nn_infer_corrs": [_G, _P, _P, _I, _P, _SZ, _P, _P],
     "tp_nn_backward": [_P, _P, _P, _P, _I, _F, _F, _F, _P, _P, _P],
     "tp_nn_adam": [_P, _P, _P, _P, _I, _F, _F, _F, _F, _I, _P, _P],
     "tp_nn_pack_bf16": [_P, _I, _P, _P],

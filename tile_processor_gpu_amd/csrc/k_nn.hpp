@@ -622,8 +622,20 @@ __device__ __forceinline__ float nn_wave_max(float v) {
 
 // 2 workgroups per CU (69 KB of LDS each): 2 waves per SIMD, at most 256
 // registers per lane
+// Where the inference reads its input: x [n][7][225] (tp_nn_input's
+// output), or, with FROM_CORRS, the correlation rows 6 task + pair (radius 7)
+// and the task words, mapped and summed as k_nn_input does.
+struct NNInferIn {
+    const float *x;
+    const tp_task *tasks;
+    const float *corrs;
+    size_t corr_stride;
+    int tiles_x, tiles_y;
+};
+
+template <bool FROM_CORRS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, int n, float *out) {
+void k_nn_infer(const uint16_t *__restrict__ prm, NNInferIn in, int n, float *out) {
     __shared__ __align__(16) uint16_t X0[NN_PROWS * 8];   // x, padded, channel 7 zero
     __shared__ __align__(16) uint16_t H1[NN_PROWS * NN_CS];
     // h2 on the padded grid as h1: the positions of the slots past the tile
@@ -655,10 +667,38 @@ void k_nn_infer(const uint16_t *__restrict__ prm, const float *__restrict__ x, i
     // channels of a tile, requested one tile ahead
     const bool pin = t < NN_P;
     float xn[NN_IN];
-    auto load_x = [&](int tile) {
-        const float *xt = x + (size_t)tile * NN_IN * NN_P + (pin ? t : 0);
+    // FROM_CORRS: offset of this pixel's source in each pair's row (-1: none)
+    int psrc[TP_NUM_PAIRS];
 #pragma unroll
-        for (int c = 0; c < NN_IN; c++) xn[c] = (tile < n && pin) ? xt[c * NN_P] : 0.f;
+    for (int p = 0; p < TP_NUM_PAIRS; p++) {
+        int sy, sx;
+        psrc[p] = (pin && pair_src(p, 7, t / 15 - 7, t % 15 - 7, sy, sx)) ? (sy + 7) * 15 + sx + 7 : -1;
+    }
+    auto load_x = [&](int tile) {
+        if (!FROM_CORRS) {
+            const float *xt = in.x + (size_t)tile * NN_IN * NN_P + (pin ? t : 0);
+#pragma unroll
+            for (int c = 0; c < NN_IN; c++) xn[c] = (tile < n && pin) ? xt[c * NN_P] : 0.f;
+        } else {
+            // as k_nn_input: pair p's mapped value if the task has the pair
+            // and the tile is on the grid, channel 0 their sum in pair order
+            int task = 0;
+            if (tile < n) {
+                gtask *tk = gtasks(in.tasks) + tile;
+                const int txy = tk->txy, tx = txy & 0xfff, ty = txy >> 16;
+                task = (tx < in.tiles_x && ty >= 0 && ty < in.tiles_y) ? tk->task : 0;
+            }
+            gfloat *rows = gptr(in.corrs) + (size_t)6 * (tile < n ? tile : 0) * in.corr_stride;
+            float sum = 0.f;
+#pragma unroll
+            for (int p = 0; p < TP_NUM_PAIRS; p++) {
+                const bool use = ((task >> (TP_TASK_CORR_BITS + p)) & 1) && psrc[p] >= 0;
+                const float v = use ? rows[p * in.corr_stride + psrc[p]] : 0.f;
+                xn[1 + p] = v;
+                sum += v;
+            }
+            xn[0] = sum;
+        }
     };
     const int xpos = pin ? NN_PW * (t / 15) + t % 15 + NN_POFF : NN_PJUNK;
     const f32x16 zero = {};

@@ -752,9 +752,33 @@ int tp_nn_infer(const uint16_t *gpu_params_bf16, const float *gpu_x, int n, floa
     if (n < 0) return fail(TP_EINVAL, "tp_nn_infer: n < 0%s", "");
     g_err.clear();
     if (!n) return TP_OK;
-    hipLaunchKernelGGL(tp::k_nn_infer, dim3(nn_grid(n)), dim3(256), 0, as_stream(stream),
-                       gpu_params_bf16, gpu_x, n, gpu_out);
+    tp::NNInferIn in{};
+    in.x = gpu_x;
+    hipLaunchKernelGGL(tp::k_nn_infer<false>, dim3(nn_grid(n)), dim3(256), 0, as_stream(stream),
+                       gpu_params_bf16, in, n, gpu_out);
     return check_launch("nn_infer");
+}
+
+int tp_nn_infer_corrs(const tp_geometry *geom, const uint16_t *gpu_params_bf16, const tp_task *gpu_tasks,
+                      int num_tasks, const float *gpu_corrs, size_t corr_stride, float *gpu_out,
+                      tp_stream_t stream) {
+    int rc = check_geometry(geom);
+    if (rc) return rc;
+    if (!gpu_params_bf16 || !gpu_tasks || !gpu_corrs || !gpu_out)
+        return fail(TP_EINVAL, "tp_nn_infer_corrs: NULL pointer argument%s", "");
+    if (num_tasks < 0) return fail(TP_EINVAL, "tp_nn_infer_corrs: num_tasks < 0%s", "");
+    if (corr_stride < 225) return fail(TP_EINVAL, "tp_nn_infer_corrs: corr_stride < 225 (radius 7 rows)%s", "");
+    g_err.clear();
+    if (!num_tasks) return TP_OK;
+    tp::NNInferIn in{};
+    in.tasks = gpu_tasks;
+    in.corrs = gpu_corrs;
+    in.corr_stride = corr_stride;
+    in.tiles_x = geom->img_width / 8;
+    in.tiles_y = geom->img_height / 8;
+    hipLaunchKernelGGL(tp::k_nn_infer<true>, dim3(nn_grid(num_tasks)), dim3(256), 0, as_stream(stream),
+                       gpu_params_bf16, in, num_tasks, gpu_out);
+    return check_launch("nn_infer_corrs");
 }
 
 int tp_nn_backward(const uint16_t *gpu_params_bf16, const float *gpu_x, const float *gpu_acts,

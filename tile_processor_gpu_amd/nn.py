@@ -105,6 +105,17 @@ class DisparityCNN:
         _lib.call("tp_nn_infer", _p(self.bf16), _p(x), int(n), _p(out), tp._stream(stream))
         return out
 
+    def infer_corrs(self, geom, tasks, n, corrs, corr_stride, out=None, stream=None):
+        """infer() on the input that tp_nn_input would assemble, read straight
+        from the correlation rows (6 task + pair, radius 7): tasks is the
+        device task list, n the number of tasks."""
+        if out is None:
+            out = torch.empty((n, 2), dtype=torch.float32, device=self.device)
+        g = geom.c()
+        _lib.call("tp_nn_infer_corrs", ctypes.byref(g), _p(self.bf16), _p(tasks), int(n), _p(corrs),
+                  int(corr_stride), _p(out), tp._stream(stream))
+        return out
+
     def gradient(self, x, target, n, acts, conf_weight=0.1, huber_delta=1.0, stream=None):
         """Gradient of the mean loss into self.grads (overwritten); returns
         the loss tensor (device, not synchronised)."""
