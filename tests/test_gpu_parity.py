@@ -335,3 +335,34 @@ def test_combine_irregular_list(small, orc, r, stride):
     has[idx >> 8] = True
     assert rel_err(got[has], ref[has]) < TOL
     assert (got[~has] == -5.0).all()
+
+
+@pytest.mark.parametrize("W,H,seed", [(144, 112, 21), (208, 160, 22), (96, 240, 23)])
+def test_frame_stages_other_geometries(orc, W, H, seed):
+    """Forward CLT, correlation (all pairs, r = 7) and whole-frame inverse CLT
+    against the FP64 oracle on tile grids of other shapes (18 x 14, 26 x 20,
+    12 x 30; image sizes are multiples of 16): list lengths, strips and bands
+    that are not multiples of the kernels' work sizes."""
+    g = Geometry(W, H)
+    cfg = synth.config_small(seed=seed, geom=g)
+    idx = corr_indices(g, cfg["tasks"])
+    print(f"[gpu parity] geometry {g}, {len(idx)} pairs", flush=True)
+    fb = tp.FrameBuffers(g, "cuda:0", corr_radius=7, max_corr=len(idx))
+    fb.upload(cfg["images"], cfg["kernels"], cfg["koffs"], cfg["tasks"], idx)
+    fb.convert()
+    fb.correlate(3, SCALES, FAT_ZERO)
+    dstride = g.img_width + 8
+    planes = [torch.zeros((3, g.img_height + 8, dstride), device="cuda:0") for _ in range(4)]
+    tp.imclt_rbg_all(g, fb.p_clt, tp.ptr_array(planes), dstride)
+    torch.cuda.synchronize()
+    clt64 = orc.convert_correct_tiles(g, cfg["koffs"], cfg["kernels"], cfg["images"], cfg["tasks"])
+    got = gpu_clt(fb)
+    for cam in range(4):
+        assert rel_err(got[cam], clt64[cam]) < TOL, cam
+    ref = orc.correlate2D(g, clt64, 3, SCALES, FAT_ZERO, idx, 7)
+    assert rel_err(fb.corrs[:len(idx), :225].cpu().numpy(), ref) < TOL
+    rplanes = orc.imclt_frame(g, clt64, dstride)
+    for cam in range(4):
+        p = planes[cam].cpu().numpy()
+        for color in range(3):
+            assert rel_err(p[color], rplanes[cam][color]) < TOL, (cam, color)
