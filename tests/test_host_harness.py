@@ -124,6 +124,17 @@ def test_harness_results_match_oracle_and_binding(files, orc):
         got = kernelgen.read_floats(str(out / f"main_chn{c}.rbg"), (3, g.img_height + 8, g.img_width + 8))
         for color in range(3):
             assert rel_err(got[color], planes[c][color]) < TOL, (c, color)
+    # the whole-frame inverse CLT (one launch) through the harness
+    out2 = d / "out_all"
+    out2.mkdir(exist_ok=True)
+    r2 = _run(["--prefix", prefix, "--out", out2, "--iterations", 1, "--fat-zero", FZ, "--imclt-all"]
+              + _geom_args(g))
+    assert r2.returncode == 0, r2.stdout[-2000:] + r2.stderr[-2000:]
+    assert "Average imclt_rbg_all run time" in r2.stdout
+    for c in range(4):
+        got = kernelgen.read_floats(str(out2 / f"main_chn{c}.rbg"), (3, g.img_height + 8, g.img_width + 8))
+        for color in range(3):
+            assert rel_err(got[color], planes[c][color]) < TOL, (c, color)
 
     # the same library through the Python binding: texture tiles, RGBA image
     fb = tp.FrameBuffers(g, "cuda:0", corr_radius=7, max_corr=len(idx))
