@@ -44,6 +44,9 @@ struct TexArgs {
     float min_shot, scale_shot, diff_sigma, diff_threshold, min_agree;
     float w[3];
     int dust_remove, keep_weights;
+    // generate_RBGA's first parity pass: the RGBA image was just cleared, so
+    // its overlap-add 0 + x is a plain store (no read of the image)
+    int first_pass;
     size_t rbga_stride;
     float *rbga;
     size_t tex_stride;
@@ -303,7 +306,10 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     const int ty = tile / a.g.TX, tx = tile - ty * a.g.TX;
     const size_t slice = a.rbga_stride * (size_t)(a.woi[3] + 1) * 8;
     float *dst = a.rbga + (size_t)((ty - a.woi[1]) * 8 + r) * a.rbga_stride + (tx - a.woi[0]) * 8 + c;
-    for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
+    if (a.first_pass)
+        for (int s = 0; s <= colors; s++) dst[s * slice] = out[s] * f;
+    else
+        for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
 }
 
 // Position in a.idx of entry e: in a parity quarter, entries past the inner

@@ -374,6 +374,7 @@ tp::TexArgs tex_args(const tp_geometry *geom, const int *woi, float **clt, const
     a.w[2] = w2;
     a.dust_remove = dust_remove;
     a.keep_weights = keep_weights;
+    a.first_pass = 0;
     a.rbga_stride = 0;
     a.rbga = nullptr;
     a.tex_stride = 0;
@@ -531,6 +532,7 @@ int tp_generate_RBGA(const tp_geometry *geom, tp_task *gpu_tasks, int num_tiles,
         p.idx = gpu_texture_indices + par * quarter;
         p.quarter = quarter;
         p.n = quarter;
+        p.first_pass = par == 0;
         if ((rc = launch_textures(p, quarter, st))) return rc;
     }
     return TP_OK;
