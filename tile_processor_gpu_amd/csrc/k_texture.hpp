@@ -149,6 +149,21 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     constexpr int colors = COLORS;
     const int t = threadIdx.x, r = px.r, c = px.c;
     const float wsq = px.wsq;
+    // The overlap-add's read of the RGBA image (written by an earlier pass)
+    // is requested here, so that its latency passes under the pixel work.
+    const int tcode = code & TP_TASK_TEXTURE_BITS;
+    const bool to_rgba = tcode && a.rbga && a.rbga_stride;
+    float *dst_rgba = nullptr;
+    size_t slice = 0;
+    float prev[4] = {0.f, 0.f, 0.f, 0.f};
+    if (to_rgba) {
+        const int ty = tile / a.g.TX, tx = tile - ty * a.g.TX;
+        slice = a.rbga_stride * (size_t)(a.woi[3] + 1) * 8;
+        dst_rgba = a.rbga + (size_t)((ty - a.woi[1]) * 8 + r) * a.rbga_stride + (tx - a.woi[0]) * 8 + c;
+        if (!a.first_pass)
+#pragma unroll
+            for (int s = 0; s <= colors; s++) prev[s] = dst_rgba[s * slice];
+    }
     // phase 2: debayer_shot (colors > 1): 3 x 3 debayer of the un-windowed
     // pixels (edge pixels unchanged), shot-noise companding, window; else
     // D = windowed U.  Thread t keeps D and U of its own pixel.
@@ -300,16 +315,10 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
         float *dst = a.tiles + (size_t)e * a.tex_stride + t;
         for (int s = 0; s < nsl; s++) dst[s * 256] = out[s];
     }
-    const int tcode = code & TP_TASK_TEXTURE_BITS;
-    if (!tcode || !a.rbga || !a.rbga_stride) return;
+    if (!to_rgba) return;
     const float f = tcode == TP_TASK_TEXTURE_BITS ? 1.f : alpha_fade(tcode, r, c);
-    const int ty = tile / a.g.TX, tx = tile - ty * a.g.TX;
-    const size_t slice = a.rbga_stride * (size_t)(a.woi[3] + 1) * 8;
-    float *dst = a.rbga + (size_t)((ty - a.woi[1]) * 8 + r) * a.rbga_stride + (tx - a.woi[0]) * 8 + c;
-    if (a.first_pass)
-        for (int s = 0; s <= colors; s++) dst[s * slice] = out[s] * f;
-    else
-        for (int s = 0; s <= colors; s++) dst[s * slice] += out[s] * f;
+#pragma unroll
+    for (int s = 0; s <= colors; s++) dst_rgba[s * slice] = a.first_pass ? out[s] * f : prev[s] + out[s] * f;
 }
 
 // Position in a.idx of entry e: in a parity quarter, entries past the inner

@@ -24,7 +24,7 @@
 #define TP_STRIP_ROWS 2        // fused kernel: tile rows per strip of the visiting order
 #define TP_TEX_WG_PER_CU 5     // textures: workgroups per CU in the (list-looping) grid
 #define TP_TEX_TPW 2           // textures: list entries (tiles) per workgroup at a time
-#define TP_TEX_WAVES 4         // textures: minimum waves per SIMD
+#define TP_TEX_WAVES 5         // textures: minimum waves per SIMD
 #define TP_COMBINE_EPW 64      // pair combination: list entries per workgroup
 
 namespace tp {
