@@ -321,28 +321,50 @@ __device__ __forceinline__ void texture_pixel(const TexArgs &a, int e, int code,
     for (int s = 0; s <= colors; s++) dst_rgba[s * slice] = a.first_pass ? out[s] * f : prev[s] + out[s] * f;
 }
 
-// Position in a.idx of entry e: in a parity quarter, entries past the inner
-// count are the border entries at the quarter's end.
-__device__ __forceinline__ int entry_pos(const TexArgs &a, int e) {
-    if (!a.quarter) return e;
-    const int n_in = min(a.num_ptr[0], a.quarter);
-    const int n_bd = min(a.num_ptr[1], a.quarter - n_in);
-    return e < n_in ? e : a.quarter - n_bd + (e - n_in);
+// The list entries of a launch: n in all; in a parity quarter the first
+// n_in are the inner entries from the quarter's start, the others the
+// border entries at its end (device counts, capped at the quarter).
+struct TexList {
+    int n, n_in, n_bd;
+};
+__device__ __forceinline__ TexList tex_list(const TexArgs &a) {
+    TexList l{a.n, a.n, 0};
+    if (a.quarter) {
+        l.n_in = min(a.num_ptr[0], a.quarter);
+        l.n_bd = min(a.num_ptr[1], a.quarter - l.n_in);
+        l.n = l.n_in + l.n_bd;
+    }
+    return l;
 }
+// Position in a.idx of entry e < l.n
+__device__ __forceinline__ int entry_pos(const TexArgs &a, const TexList &l, int e) {
+    if (!a.quarter) return e;
+    return e < l.n_in ? e : a.quarter - l.n_bd + (e - l.n_in);
+}
+// List words of entries e0 .. e0 + TPW - 1 (0 past the end): lane k < TPW
+// loads entry e0 + k's with a vector load, which the caller reads with
+// list_word once it is needed (so a wait for it is not also a wait for LDS
+// operations, as a scalar load's would be).
+template <int TPW>
+__device__ __forceinline__ int list_words(const TexArgs &a, const TexList &l, int e0) {
+    const int e = e0 + min((int)__lane_id(), TPW - 1);
+    return e < l.n ? a.idx[entry_pos(a, l, e)] : 0;
+}
+__device__ __forceinline__ int list_word(int words, int k) { return __builtin_amdgcn_readlane(words, k); }
 
 // List entries e0 .. e0 + TPW - 1 (of n) by the whole workgroup:
 // phase 1 runs the units of all its tiles at once (8 lanes each), then the
 // threads take the tiles' pixels one tile after the other.  Entries without
 // a texture bit, off the grid or past n are skipped (uniformly).
 template <int COLORS, int TPW, bool KEEPW>
-__device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int n, TexShared<TPW> &sh,
+__device__ __forceinline__ void texture_tiles(const TexArgs &a, int e0, int words, TexShared<TPW> &sh,
                                               const TexPix &px) {
     constexpr int colors = COLORS, UPT = 4 * COLORS;  // units per tile
     int code[TPW], tile[TPW];
     bool ok[TPW], any = false;
 #pragma unroll
     for (int k = 0; k < TPW; k++) {
-        code[k] = e0 + k < n ? a.idx[entry_pos(a, e0 + k)] : 0;
+        code[k] = list_word(words, k);
         tile[k] = code[k] >> TP_CORR_NTILE_SHIFT;
         ok[k] = (code[k] & (1 << TP_LIST_TEXTURE_BIT)) && tile[k] >= 0 && tile[k] < a.g.TX * a.g.TY;
         any = any || ok[k];
@@ -388,17 +410,18 @@ template <int COLORS, int TPW, bool KEEPW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_TEX_WAVES)))
 void k_textures_accumulate(TexArgs a) {
     __shared__ __align__(16) TexShared<TPW> sh;
-    // device counts are capped at the quarter
-    int n = a.n;
-    if (a.quarter) {
-        const int n_in = min(a.num_ptr[0], a.quarter);
-        n = n_in + min(a.num_ptr[1], a.quarter - n_in);
-    }
+    const TexList l = tex_list(a);
     const TexPix px = tex_pix(threadIdx.x, a.w);
     if (threadIdx.x == 0) tex_constants(a, sh.cst);
     __syncthreads();
-    for (int e = blockIdx.x * TPW; e < n; e += gridDim.x * TPW) {
-        texture_tiles<COLORS, TPW, KEEPW>(a, e, n, sh, px);
+    // the next entries' list words are requested a round ahead: the CLT
+    // loads that start each round then wait for one memory latency, not two
+    const int step = gridDim.x * TPW;
+    int words = list_words<TPW>(a, l, blockIdx.x * TPW);
+    for (int e = blockIdx.x * TPW; e < l.n; e += step) {
+        const int cur = words;
+        words = list_words<TPW>(a, l, e + step);
+        texture_tiles<COLORS, TPW, KEEPW>(a, e, cur, sh, px);
         __syncthreads();  // the LDS of these entries is rewritten by the next ones
     }
 }
