@@ -6,9 +6,8 @@
 // Work decomposition (gfx950, wave64): a "unit" is one (tile, camera, color)
 // 16x16 -> 4 x 8x8 transform.  8 lanes own a unit (lane = one row or column
 // of the 8x8 folded block), so a wavefront runs 8 units side by side.  The
-// staged kernel k_convert_cam gives a workgroup 8 neighbouring tiles of one
-// tile row and its 4 wavefronts the 4 cameras of them; a wavefront converts
-// R, B, then G of its 8 tiles.
+// staged kernel k_convert_cam gives a workgroup 32 consecutive tiles of one
+// camera, 8 per wavefront; a wavefront converts R, B, then G of its 8 tiles.
 //
 // Per unit:
 //   1. lane = folded column k gathers its 8 (R, B) or 16 (G) Bayer samples,
@@ -634,23 +633,41 @@ __device__ __forceinline__ void rgb_units(const ConvertArgs &la, const LaneFold 
     store_block(slice, dst, li, st);
 }
 
-// Staged forward CLT: workgroup b converts tiles 8 b .. 8 b + 7 of the task
-// list (8 neighbouring tiles of a tile row for a raster list), wavefront w
-// camera w of them.  A wavefront's 8 units are 8 neighbouring tiles of one
-// camera, so units that share an aberration kernel cell (2 x 2 tiles per
-// cell) or image lines load the same addresses in the same instruction, and
-// the camera's pointers are wavefront-uniform.  Slots past the end of the
-// list convert the last task and store nothing.
+// Blocks of k_convert_cam per camera: patches of 8 list columns x 4 list
+// rows of TX entries (n >= 1).
+__host__ __device__ inline int convert_blocks(int n, int TX) {
+    return ((n - 1) / TX / 4 + 1) * ((TX + 7) / 8);
+}
+
+// Staged forward CLT.  The list is read as rows of TX entries (a raster
+// list: tile rows); workgroup b converts a patch of 8 columns x 4 rows of it
+// for camera b / nb (nb = convert_blocks), wavefront w row w of the patch.
+// The grid runs camera after camera, patch row after patch row, so the
+// workgroups resident at a time write one contiguous stretch of one
+// camera's CLT (3 KB per tile) and read one camera's image and kernels,
+// rather than four stretches in the four cameras' buffers, and the 4 tile
+// rows of a patch share kernel cells and image lines within the workgroup.
+// A wavefront's 8 units are 8 neighbouring tiles of one camera, so units
+// that share an aberration kernel cell (2 x 2 tiles per cell) or image lines
+// load the same addresses in the same instruction, and the camera's pointers
+// are uniform.  Slots past the end of a row or of the list convert the last
+// task and store nothing.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TP_CONVERT_WAVES)))
 void k_convert_cam(ConvertArgs a) {
     __shared__ __align__(16) float lds[4][8][kSlice];  // [wave][unit]: staging, then output block
-    const int cam = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6;
+    const int TX = a.g.TX, cpr = (TX + 7) / 8;  // 8-entry column groups per list row
+    const int nb = convert_blocks(a.num_tiles, TX);
+    const int cam = __builtin_amdgcn_readfirstlane((int)blockIdx.x / nb);
+    const int blk = (int)blockIdx.x - cam * nb;
+    const int prow = blk / cpr, pcol = blk - prow * cpr;
     const int lane = threadIdx.x & 63;
     const int unit = lane >> 3;
     const int li = lane & 7;
-    const int s = xcd_remap(blockIdx.x, gridDim.x) * 8 + unit;
-    const bool valid = s < a.num_tiles;
-    float *slice = &lds[cam][unit][0];
+    const int col = pcol * 8 + unit;
+    const int s = (prow * 4 + wave) * TX + col;
+    const bool valid = col < TX && s < a.num_tiles;
+    float *slice = &lds[wave][unit][0];
     const LaneFold lf = lane_fold(li);
     ConvertArgs la = a;
     resolve_cam(la, cam);

@@ -245,8 +245,11 @@ int tp_convert_correct_tiles(const tp_geometry *geom, float **gpu_kernel_offsets
     a.dstride = dstride;
     a.num_tiles = num_tiles;
     a.lpf_mask = lpf_mask;
-    // one workgroup (4 wavefronts, one per camera) per block of 8 tasks
-    hipLaunchKernelGGL(tp::k_convert_cam, dim3((num_tiles + 7) / 8), dim3(256), 0, as_stream(stream), a);
+    // one workgroup (4 wavefronts) per camera and patch of 8 x 4 list
+    // entries, camera-major
+    const long long grid = 4LL * tp::convert_blocks(num_tiles, a.g.TX);
+    if (grid > 0x7fffffffLL) return fail(TP_EINVAL, "tp_convert_correct_tiles: too many tiles%s", "");
+    hipLaunchKernelGGL(tp::k_convert_cam, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), a);
     return check_launch("convert_correct_tiles");
 }
 
