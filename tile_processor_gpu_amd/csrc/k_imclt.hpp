@@ -170,8 +170,9 @@ void k_imclt_strip(Geo g, const float *const *clt, float *const *rbg, size_t dst
     const int unit = lane >> 3, li = lane & 7;
     const int PW = g.W + 8, PH = g.H + 8;
     const int nstrip = gridDim.x, nband = gridDim.y;
-    const int lin = xcd_remap(blockIdx.x + nstrip * (blockIdx.y + nband * blockIdx.z),
-                              nstrip * nband * gridDim.z);
+    // plane-major, strips innermost, in dispatch order (not remapped per XCD:
+    // one front over the plane is 1% faster than an eighth of it per XCD)
+    const int lin = blockIdx.x + nstrip * (blockIdx.y + nband * blockIdx.z);
     const int strip = lin % nstrip, rest = lin / nstrip;
     const int bnd = rest % nband, pidx = rest / nband;
     const int color = pidx % TP_NUM_COLORS, cam = pidx / TP_NUM_COLORS;
